@@ -1,0 +1,238 @@
+"""MI355X FTLM engine for the extended Hubbard model -- Python binding.
+
+A thin ctypes layer over libftlm_hip.so (the C ABI of include/ftlm.h). The
+engine runs on the GPU only: loading fails loudly when the library is not
+built, and creating an Engine fails when no HIP device is usable. There is no
+CPU fallback.
+
+The Engine methods mirror the reference's hot-path seams (src/main.cpp):
+build_sector_basis (622-721) -> sector, build_momentum_block (723-805) ->
+block, apply_hamiltonian (833-845) -> apply, run_ftlm_block (996-1113) ->
+ftlm_block, diagonalize_block_exact (847-911) -> exact_blocks.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libftlm_hip.so")
+HOST_LIB_PATH = os.path.join(PKG_DIR, "libftlm_host.so")
+FTLM_EXE = os.path.join(PKG_DIR, "ftlm_n_vs_mu")
+ED_EXE = os.path.join(PKG_DIR, "ftlm_ed_n_vs_mu")
+
+REDUCE_SERIAL = 0
+REDUCE_TREE = 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_up = C.POINTER(C.c_uint64)
+
+
+class FtlmModel(C.Structure):
+    _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("tx", C.c_double), ("ty", C.c_double),
+                ("tp", C.c_double), ("phix", C.c_double), ("phiy", C.c_double),
+                ("u", C.c_double), ("v", C.c_double)]
+
+
+class FtlmStats(C.Structure):
+    _fields_ = [("ms_apply", C.c_double), ("ms_total", C.c_double), ("n_apply", C.c_int64),
+                ("bytes_apply", C.c_double)]
+
+
+# every symbol declared in include/ftlm.h, with its ctypes signature
+SIGNATURES = {
+    "ftlm_last_error": (C.c_char_p, []),
+    "ftlm_block_seed": (C.c_uint64, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ftlm_create": (C.c_int, [C.POINTER(FtlmModel), C.c_int, C.POINTER(C.c_void_p)]),
+    "ftlm_destroy": (None, [C.c_void_p]),
+    "ftlm_sector": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _up, _lp]),
+    "ftlm_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _lp]),
+    "ftlm_block_reps": (C.c_int, [C.c_void_p, _up, _up, _ip, _dp]),
+    "ftlm_block_csr": (C.c_int, [C.c_void_p, C.c_int, _lp, _ip, _ip, _dp, _dp]),
+    "ftlm_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
+    "ftlm_start_vector": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, _dp]),
+    "ftlm_lanczos": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, C.c_uint64, C.c_int, C.c_int, _dp, _dp, _ip]),
+    "ftlm_ritz": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp]),
+    "ftlm_ftlm_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, _dp, _dp, _lp]),
+    "ftlm_prepare": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
+    "ftlm_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ftlm_finish": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _lp]),
+    "ftlm_last_stats": (C.c_int, [C.c_void_p, C.POINTER(FtlmStats)]),
+    "ftlm_exact_blocks": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _lp]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libftlm_hip.so (raises if it is not built: no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class FtlmError(RuntimeError):
+    pass
+
+
+def block_seed(seed, nup, ndn, mx, my):
+    """make_block_seed (main.cpp:520-533)."""
+    return load_library().ftlm_block_seed(seed, nup, ndn, mx, my)
+
+
+class Engine:
+    """One GPU context: a lattice model, a current sector and block."""
+
+    def __init__(self, lx=2, ly=2, tx=1.0, ty=1.0, tp=0.0, phix=1.0, phiy=1.0, u=8.0, v=0.0, device=0):
+        self.lib = load_library()
+        self.model = FtlmModel(lx, ly, tx, ty, tp, phix, phiy, u, v)
+        self.L = lx * ly
+        h = C.c_void_p()
+        self._check(self.lib.ftlm_create(C.byref(self.model), device, C.byref(h)))
+        self.h = h
+        self.nup = self.ndn = None
+        self.dim = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FtlmError(f"ftlm error {rc}: {self.lib.ftlm_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ftlm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sector(self, nup, ndn):
+        fd = C.c_uint64(0)
+        par = C.c_int64(0)
+        self._check(self.lib.ftlm_sector(self.h, nup, ndn, C.byref(fd), C.byref(par)))
+        self.nup, self.ndn = nup, ndn
+        return fd.value, par.value
+
+    def block(self, mx, my):
+        d = C.c_int64(0)
+        self._check(self.lib.ftlm_block(self.h, mx, my, C.byref(d)))
+        self.dim = d.value
+        return self.dim
+
+    def reps(self):
+        n = self.dim
+        up = np.zeros(n, np.uint64)
+        dn = np.zeros(n, np.uint64)
+        deg = np.zeros(n, np.int32)
+        dnf = np.zeros(n, np.float64)
+        self._check(self.lib.ftlm_block_reps(self.h, _p(up, _up), _p(dn, _up), _p(deg, _ip), _p(dnf, _dp)))
+        return up, dn, deg, dnf
+
+    def csr(self, all_dirs=True):
+        nnz = C.c_int64(0)
+        self._check(self.lib.ftlm_block_csr(self.h, int(all_dirs), C.byref(nnz), None, None, None, None))
+        n = self.dim
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1), np.complex128)
+        diag = np.zeros(max(n, 1), np.float64)
+        self._check(self.lib.ftlm_block_csr(self.h, int(all_dirs), C.byref(nnz), _p(rp, _ip), _p(col, _ip),
+                                            _p(val, _dp), _p(diag, _dp)))
+        return rp, col[: nnz.value], val[: nnz.value], diag[:n]
+
+    def apply(self, x, kernel=0):
+        """x: complex (dim,) or (dim, R) -> H x (same shape)."""
+        x2 = np.ascontiguousarray(np.asarray(x).reshape(self.dim, -1), dtype=np.complex128)
+        y = np.zeros_like(x2)
+        self._check(self.lib.ftlm_apply(self.h, x2.shape[1], kernel, _p(x2, _dp), _p(y, _dp)))
+        return y.reshape(np.shape(x))
+
+    def start_vector(self, bseed, sample):
+        out = np.zeros(self.dim, np.complex128)
+        self._check(self.lib.ftlm_start_vector(self.h, bseed, sample, _p(out, _dp)))
+        return out
+
+    def lanczos(self, R, steps, start=None, bseed=0, reduce=REDUCE_TREE, per_vector=False):
+        """alpha [steps][R], beta [steps][R], m [R] of R vectors."""
+        a = np.zeros((steps, R))
+        b = np.zeros((steps, R))
+        m = np.zeros(R, np.int32)
+        sp = None
+        if start is not None:
+            start = np.ascontiguousarray(np.asarray(start).reshape(self.dim, R), dtype=np.complex128)
+            sp = _p(start, _dp)
+        self._check(self.lib.ftlm_lanczos(self.h, R, steps, sp, bseed, reduce, int(per_vector), _p(a, _dp),
+                                          _p(b, _dp), _p(m, _ip)))
+        return a, b, m
+
+    def ritz(self, alpha, beta, m, wdiv):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        beta = np.ascontiguousarray(beta, np.float64)
+        m = np.ascontiguousarray(m, np.int32)
+        steps, R = alpha.shape
+        e = np.zeros((R, steps))
+        w = np.zeros((R, steps))
+        self._check(self.lib.ftlm_ritz(self.h, R, steps, _p(alpha, _dp), _p(beta, _dp), _p(m, _ip), wdiv,
+                                       _p(e, _dp), _p(w, _dp)))
+        return e, w
+
+    def ftlm_block(self, R, steps, bseed, reduce=REDUCE_TREE, per_vector=False):
+        n = R * min(steps, self.dim)
+        e = np.zeros(max(n, 1))
+        w = np.zeros(max(n, 1))
+        cnt = C.c_int64(0)
+        self._check(self.lib.ftlm_ftlm_block(self.h, R, steps, bseed, reduce, int(per_vector), _p(e, _dp),
+                                             _p(w, _dp), C.byref(cnt)))
+        return e[: cnt.value], w[: cnt.value]
+
+    def prepare(self, R, bseed):
+        self._check(self.lib.ftlm_prepare(self.h, R, bseed))
+
+    def run(self, R, steps, reduce=REDUCE_TREE, per_vector_column=-1):
+        self._check(self.lib.ftlm_run(self.h, R, steps, reduce, per_vector_column))
+        s = FtlmStats()
+        self._check(self.lib.ftlm_last_stats(self.h, C.byref(s)))
+        return {"ms_apply": s.ms_apply, "ms_total": s.ms_total, "n_apply": s.n_apply,
+                "bytes_apply": s.bytes_apply}
+
+    def finish(self, R, steps):
+        n = R * min(steps, self.dim)
+        e = np.zeros(max(n, 1))
+        w = np.zeros(max(n, 1))
+        cnt = C.c_int64(0)
+        self._check(self.lib.ftlm_finish(self.h, R, steps, _p(e, _dp), _p(w, _dp), C.byref(cnt)))
+        return e[: cnt.value], w[: cnt.value]
+
+    def exact_blocks(self, ks):
+        """ks: list of (mx, my) of the current sector -> list of eigenvalue arrays."""
+        mx = np.array([k[0] for k in ks], np.int32)
+        my = np.array([k[1] for k in ks], np.int32)
+        dims = []
+        for a, b in ks:
+            dims.append(self.block(int(a), int(b)))
+        tot = int(sum(dims))
+        eig = np.zeros(max(tot, 1))
+        off = np.zeros(len(ks) + 1, np.int64)
+        self._check(self.lib.ftlm_exact_blocks(self.h, len(ks), _p(mx, _ip), _p(my, _ip), _p(eig, _dp),
+                                               _p(off, _lp)))
+        return [eig[off[i]:off[i + 1]].copy() for i in range(len(ks))]

@@ -1,0 +1,505 @@
+// ftlm_apply.hip -- on-the-fly Hamiltonian application (gfx950).
+//
+// y[i][r] = diag(i) x[i][r] + sum_hops coef(i,h) x[col(i,h)][r]
+// for R column vectors stored row-major [dim][R] complex double, the layout
+// in which one gathered row is R*16 contiguous bytes. This is apply_hamiltonian
+// (main.cpp:833-845) for all R random vectors of a block at once, with the
+// Hamiltonian never stored: each hop's target row and coefficient are
+// recomputed from the basis tables (main.cpp:677-718, 783-801).
+//
+// Rounding matches the reference: the row sum starts with 0 + diag*x, adds
+// the hops in the reference's order (up electrons by ascending site, then
+// down electrons, directions 0..7) and the build uses -ffp-contract=off.
+// Hops along directions with zero hopping are skipped; they contribute an
+// exact zero there.
+//
+// Batched kernel (k_apply): a persistent grid walks tiles of `rpt` rows.
+//   phase A  hop metadata of the tile -> LDS (one thread per row x electron)
+//   phase B  gather: `lpr` lanes per row, each lane owns `vpl` columns
+//            (r = lane + v*lpr), so every gathered row is read as fully
+//            coalesced 16-byte lanes.
+// Tiles are split into 8 contiguous ranges, one per XCD (workgroup g runs on
+// XCD g % 8), so rows that share an up configuration -- whose down-spin hops
+// land within the same few MB -- are processed under one L2.
+#include "ftlm_device.h"
+#include "ftlm_launch.h"
+
+namespace ftlm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kXcd = 8;
+
+struct SharedTables {
+  LatticeConst g;
+};
+
+// copy lattice constants, binomial rows and the coefficient table to LDS
+__device__ __forceinline__ void stage_tables(const LatticeConst* gp, const BlockView& bv, int kmax,
+                                             LatticeConst& sg, uint64_t* sbinom, double* sptab) {
+  const int tid = threadIdx.x;
+  const int nwords = (int)(sizeof(LatticeConst) / 8);
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(gp);
+  uint64_t* dst = reinterpret_cast<uint64_t*>(&sg);
+  for (int i = tid; i < nwords; i += blockDim.x) dst[i] = src[i];
+  for (int i = tid; i < (kmax + 1) * 64; i += blockDim.x) sbinom[i] = bv.binom[i];
+  __syncthreads();
+  for (int i = tid; i < 16 * sg.L; i += blockDim.x) sptab[i] = bv.ptab[i];
+  __syncthreads();
+}
+
+// 8-byte words of LDS taken by the staged binomial rows and coefficient table
+__host__ __device__ __forceinline__ int table_words(int kmax, int L) {
+  const int w = (kmax + 1) * 64 + 16 * L;
+  return (w + 1) & ~1;
+}
+
+__device__ __forceinline__ int select_bit(uint64_t b, int k) {
+  for (int t = 0; t < k; ++t) b &= b - 1;
+  return ctz(b);
+}
+
+template <int VPL>
+__global__ void __launch_bounds__(kThreads) k_apply(const LatticeConst* __restrict__ gp, BlockView bv, int R,
+                                                    int lpr, int rpt, int nc, int64_t ntiles,
+                                                    const double2* __restrict__ x, double2* __restrict__ y,
+                                                    double* __restrict__ partials, unsigned* counter,
+                                                    double* __restrict__ dot_out, int L) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ LatticeConst sg;
+  const int kmax = (bv.nup > bv.ndn ? bv.nup : bv.ndn) + 1;
+  uint64_t* sbinom = reinterpret_cast<uint64_t*>(smem);
+  double* sptab = reinterpret_cast<double*>(sbinom + (kmax + 1) * 64);
+  double2* scoef = reinterpret_cast<double2*>(reinterpret_cast<uint64_t*>(smem) + table_words(kmax, L));
+  int32_t* scol = reinterpret_cast<int32_t*>(scoef + (size_t)rpt * nc);
+  double* sdiag = reinterpret_cast<double*>(scol + (((size_t)rpt * nc + 1) & ~(size_t)1));
+  double* sdnf = sdiag + rpt;
+  uint64_t* sup = reinterpret_cast<uint64_t*>(sdnf + rpt);
+  uint64_t* sdn = sup + rpt;
+  int32_t* sc = reinterpret_cast<int32_t*>(sdn + rpt);
+  stage_tables(gp, bv, kmax, sg, sbinom, sptab);
+  BlockView lv = bv;
+  lv.binom = sbinom;
+  lv.ptab = sptab;
+
+  const int tid = threadIdx.x;
+  const int ne = bv.nup + bv.ndn;
+  const int ndir = sg.ndir;
+  const int rl = tid / lpr;   // row within the tile (phase B)
+  const int lane = tid % lpr;
+  double part[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) part[v] = 0.0;
+
+  // XCD-contiguous tile ranges
+  int64_t t0 = 0, t1 = ntiles, tstep = gridDim.x, tfirst = blockIdx.x;
+  if (gridDim.x >= kXcd) {
+    const int xcd = blockIdx.x % kXcd;
+    const int l = blockIdx.x / kXcd;
+    const int nwg = (gridDim.x - xcd + kXcd - 1) / kXcd;
+    const int64_t chunk = (ntiles + kXcd - 1) / kXcd;
+    t0 = xcd * chunk;
+    t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+    tfirst = t0 + l;
+    tstep = nwg;
+  }
+  for (int64_t tile = tfirst; tile < t1; tile += tstep) {
+    const int64_t row0 = tile * rpt;
+    // phase A0: row states
+    if (tid < rpt) {
+      const int64_t row = row0 + tid;
+      if (row < bv.dim) {
+        const RowState s = row_state(lv, sg, row);
+        sup[tid] = s.up;
+        sdn[tid] = s.dn;
+        sdnf[tid] = s.dnf;
+        sc[tid] = s.c;
+        sdiag[tid] = diag_energy(s.up, s.dn, sg);
+      }
+    }
+    __syncthreads();
+    // phase A1: hop metadata, one thread per (row, electron)
+    for (int t = tid; t < rpt * ne; t += kThreads) {
+      const int r = t / ne, e = t - r * ne;
+      const int64_t row = row0 + r;
+      if (row >= bv.dim) continue;
+      const uint64_t up = sup[r], dn = sdn[r];
+      const int spin = e >= bv.nup;
+      const uint64_t bits = spin ? dn : up;
+      const int from = select_bit(bits, spin ? e - bv.nup : e);
+      for (int di = 0; di < ndir; ++di) {
+        const int dir = sg.dirs[di];
+        const int to = sg.nb[from][dir];
+        const int slot = r * nc + e * ndir + di;
+        int64_t col = -1;
+        cd cf = {0.0, 0.0};
+        if (!((bits >> to) & 1ull)) {
+          if (!hop_target(lv, sg, sc[r], up, dn, sdnf[r], spin, from, to, dir, &col, &cf)) col = -1;
+        }
+        scol[slot] = (int32_t)col;
+        scoef[slot] = make_double2(cf.re, cf.im);
+      }
+    }
+    __syncthreads();
+    // phase B: gather-accumulate
+    if (rl < rpt) {
+      const int64_t row = row0 + rl;
+      if (row < bv.dim) {
+        const double dg = sdiag[rl];
+        cd acc[VPL];
+        cd xs[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int r = lane + v * lpr;
+          xs[v] = {0.0, 0.0};
+          if (r < R) {
+            const double2 t = x[row * R + r];
+            xs[v] = {t.x, t.y};
+          }
+          acc[v] = {0.0 + dg * xs[v].re, 0.0 + dg * xs[v].im};
+        }
+        const int32_t* mycol = scol + rl * nc;
+        const double2* mycf = scoef + rl * nc;
+        int s = 0;
+        for (; s + 4 <= nc; s += 4) {
+          int32_t cl[4];
+          double2 cf[4];
+          double2 xv[4][VPL];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cl[u] = mycol[s + u];
+            cf[u] = mycf[s + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+              const int r = lane + v * lpr;
+              xv[u][v] = (cl[u] >= 0 && r < R) ? x[(int64_t)cl[u] * R + r] : make_double2(0.0, 0.0);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (cl[u] < 0) continue;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v)
+              acc[v] = cadd(acc[v], cmul({cf[u].x, cf[u].y}, {xv[u][v].x, xv[u][v].y}));
+          }
+        }
+        for (; s < nc; ++s) {
+          const int32_t cl = mycol[s];
+          if (cl < 0) continue;
+          const double2 cf = mycf[s];
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            const int r = lane + v * lpr;
+            if (r < R) {
+              const double2 t = x[(int64_t)cl * R + r];
+              acc[v] = cadd(acc[v], cmul({cf.x, cf.y}, {t.x, t.y}));
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int r = lane + v * lpr;
+          if (r < R) {
+            y[row * R + r] = make_double2(acc[v].re, acc[v].im);
+            part[v] += xs[v].re * acc[v].re + xs[v].im * acc[v].im;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!partials) return;
+  // per-column partial sums of Re<x, y>: rows of the tile first, then the
+  // workgroups in index order in the last workgroup to finish
+  double* red = reinterpret_cast<double*>(scoef);
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) red[v * kThreads + tid] = part[v];
+  __syncthreads();
+  for (int r = tid; r < R; r += kThreads) {
+    const int ln = r % lpr, v = r / lpr;
+    double s = 0.0;
+    for (int q = 0; q < rpt; ++q) s += red[v * kThreads + q * lpr + ln];
+    partials[(int64_t)blockIdx.x * R + r] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (tid == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = tid; r < R; r += kThreads) {
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
+    dot_out[r] = s;
+  }
+  if (tid == 0) *counter = 0u;
+}
+
+// One row per thread: the per-vector (R = 1) Hamiltonian application.
+__global__ void __launch_bounds__(kThreads) k_apply_r1(const LatticeConst* __restrict__ gp, BlockView bv,
+                                                       const double2* __restrict__ x, double2* __restrict__ y,
+                                                       double* __restrict__ partials, unsigned* counter,
+                                                       double* __restrict__ dot_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ LatticeConst sg;
+  const int kmax = (bv.nup > bv.ndn ? bv.nup : bv.ndn) + 1;
+  uint64_t* sbinom = reinterpret_cast<uint64_t*>(smem);
+  double* sptab = reinterpret_cast<double*>(sbinom + (kmax + 1) * 64);
+  stage_tables(gp, bv, kmax, sg, sbinom, sptab);
+  BlockView lv = bv;
+  lv.binom = sbinom;
+  lv.ptab = sptab;
+  double part = 0.0;
+  const int ndir = sg.ndir;
+  for (int64_t row = blockIdx.x * (int64_t)kThreads + threadIdx.x; row < bv.dim;
+       row += (int64_t)gridDim.x * kThreads) {
+    const RowState st = row_state(lv, sg, row);
+    const double2 xt = x[row];
+    const cd xs = {xt.x, xt.y};
+    const double dg = diag_energy(st.up, st.dn, sg);
+    cd acc = {0.0 + dg * xs.re, 0.0 + dg * xs.im};
+    for (int spin = 0; spin < 2; ++spin) {
+      uint64_t occ = spin ? st.dn : st.up;
+      const uint64_t bits = occ;
+      while (occ) {
+        const int from = ctz(occ);
+        occ &= occ - 1;
+        for (int di = 0; di < ndir; ++di) {
+          const int dir = sg.dirs[di];
+          const int to = sg.nb[from][dir];
+          if ((bits >> to) & 1ull) continue;
+          int64_t col;
+          cd cf;
+          if (!hop_target(lv, sg, st.c, st.up, st.dn, st.dnf, spin, from, to, dir, &col, &cf)) continue;
+          const double2 t = x[col];
+          acc = cadd(acc, cmul(cf, {t.x, t.y}));
+        }
+      }
+    }
+    y[row] = make_double2(acc.re, acc.im);
+    part += xs.re * acc.re + xs.im * acc.im;
+  }
+  if (!partials) return;
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kThreads; ++i) s += red[i];
+    partials[blockIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[b]);
+    dot_out[0] = s;
+    *counter = 0u;
+  }
+}
+
+// Row-wise hop enumeration for CSR export and dense builds.
+template <typename F>
+__device__ __forceinline__ void for_each_hop(const BlockView& bv, const LatticeConst& g, const RowState& st,
+                                             int all_dirs, F&& f) {
+  for (int spin = 0; spin < 2; ++spin) {
+    uint64_t occ = spin ? st.dn : st.up;
+    const uint64_t bits = occ;
+    while (occ) {
+      const int from = ctz(occ);
+      occ &= occ - 1;
+      const int nd = all_dirs ? 8 : g.ndir;
+      for (int di = 0; di < nd; ++di) {
+        const int dir = all_dirs ? di : g.dirs[di];
+        const int to = g.nb[from][dir];
+        if ((bits >> to) & 1ull) continue;
+        int64_t col;
+        cd cf;
+        if (!hop_target(bv, g, st.c, st.up, st.dn, st.dnf, spin, from, to, dir, &col, &cf)) continue;
+        f(col, cf);
+      }
+    }
+  }
+}
+
+__global__ void k_csr_count(const LatticeConst* __restrict__ gp, BlockView bv, int all_dirs, int32_t* counts) {
+  const LatticeConst& g = *gp;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= bv.dim) return;
+  const RowState st = row_state(bv, g, row);
+  int n = 0;
+  for_each_hop(bv, g, st, all_dirs, [&](int64_t, cd) { ++n; });
+  counts[row] = n;
+}
+
+__global__ void k_csr_fill(const LatticeConst* __restrict__ gp, BlockView bv, int all_dirs,
+                           const int64_t* __restrict__ row_ptr, int32_t* col, double* val, double* diag) {
+  const LatticeConst& g = *gp;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= bv.dim) return;
+  const RowState st = row_state(bv, g, row);
+  diag[row] = diag_energy(st.up, st.dn, g);
+  int64_t e = row_ptr[row];
+  for_each_hop(bv, g, st, all_dirs, [&](int64_t c, cd v) {
+    col[e] = (int32_t)c;
+    val[2 * e] = v.re;
+    val[2 * e + 1] = v.im;
+    ++e;
+  });
+}
+
+// Dense column-major matrices (A[i + j*n], row i = source state) of a batch
+// of blocks, then the reference's Hermitisation (main.cpp:862-886).
+__global__ void k_dense_rows(const LatticeConst* __restrict__ gp, const BlockView* __restrict__ bvs,
+                             const int64_t* __restrict__ moff, double* mats, int64_t max_dim) {
+  const LatticeConst& g = *gp;
+  const int b = blockIdx.y;
+  const BlockView bv = bvs[b];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= bv.dim) return;
+  const int64_t n = bv.dim;
+  double* A = mats + 2 * moff[b];
+  const RowState st = row_state(bv, g, i);
+  A[2 * (i + i * n)] = diag_energy(st.up, st.dn, g);
+  A[2 * (i + i * n) + 1] = 0.0;
+  for_each_hop(bv, g, st, 0, [&](int64_t j, cd v) {
+    double* a = A + 2 * (i + j * n);
+    a[0] = a[0] + v.re;
+    a[1] = a[1] + v.im;
+  });
+  (void)max_dim;
+}
+
+__global__ void k_hermitise(const BlockView* __restrict__ bvs, const int64_t* __restrict__ moff, double* mats) {
+  const int b = blockIdx.y;
+  const int64_t n = bvs[b].dim;
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double* A = mats + 2 * moff[b];
+  A[2 * (j + j * n) + 1] = 0.0;
+  for (int64_t i = 0; i < j; ++i) {
+    double* aij = A + 2 * (i + j * n);
+    double* aji = A + 2 * (j + i * n);
+    const double re = 0.5 * (aij[0] + aji[0]);
+    const double im = 0.5 * (aij[1] - aji[1]);
+    aij[0] = re;
+    aij[1] = im;
+    aji[0] = re;
+    aji[1] = -im;
+  }
+}
+
+__global__ void k_zero(double* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+size_t table_lds_bytes(int nup, int ndn, int L) {
+  const int kmax = (nup > ndn ? nup : ndn) + 1;
+  return (size_t)table_words(kmax, L) * 8;
+}
+
+}  // namespace
+
+ApplyPlan plan_apply(const LatticeConst& g, int nup, int ndn, int64_t dim, int R, int cus) {
+  ApplyPlan p{};
+  p.R = R;
+  p.L = g.L;
+  p.vpl = R <= 64 ? 1 : (R <= 128 ? 2 : 4);
+  int need = (R + p.vpl - 1) / p.vpl;
+  int lpr = 1;
+  while (lpr < need) lpr <<= 1;
+  if (lpr > 64) lpr = 64;
+  p.lpr = lpr;
+  // hop candidate slots per row; 0 when no electron can move (an empty
+  // sector or all hoppings zero): phase B then gathers nothing
+  p.nc = (nup + ndn) * g.ndir;
+  const int ncs = p.nc > 0 ? p.nc : 1;
+  int rpt = kThreads / lpr;
+  // keep the tile's LDS (hop slots + row states) within 48 KB
+  const size_t per_row = (size_t)ncs * (16 + 4) + 48;
+  while (rpt > 1 && per_row * rpt > 48 * 1024) rpt >>= 1;
+  p.rpt = rpt;
+  const int64_t ntiles = (dim + rpt - 1) / rpt;
+  int64_t grid = (int64_t)cus * 8;
+  if (grid > ntiles) grid = ntiles;
+  if (grid < 1) grid = 1;
+  p.grid = (int)grid;
+  size_t slots = (size_t)rpt * ncs;
+  size_t lds = table_lds_bytes(nup, ndn, g.L) + slots * 16 + ((slots + 1) & ~(size_t)1) * 4 + (size_t)rpt * (8 * 4 + 4) + 64;
+  const size_t red = (size_t)p.vpl * kThreads * 8;
+  if (slots * 16 < red) lds += red - slots * 16;
+  p.lds = lds;
+  return p;
+}
+
+void launch_apply(const ApplyPlan& p, const LatticeConst* g, BlockView bv, const double* x, double* y,
+                  double* partials, unsigned* counter, double* dot_out, hipStream_t s) {
+  if (bv.dim == 0) return;
+  const int64_t ntiles = (bv.dim + p.rpt - 1) / p.rpt;
+  const int bv_L = p.L;
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  double2* yy = reinterpret_cast<double2*>(y);
+  if (p.vpl == 1)
+    k_apply<1><<<p.grid, kThreads, p.lds, s>>>(g, bv, p.R, p.lpr, p.rpt, p.nc, ntiles, xx, yy, partials, counter,
+                                               dot_out, bv_L);
+  else if (p.vpl == 2)
+    k_apply<2><<<p.grid, kThreads, p.lds, s>>>(g, bv, p.R, p.lpr, p.rpt, p.nc, ntiles, xx, yy, partials, counter,
+                                               dot_out, bv_L);
+  else
+    k_apply<4><<<p.grid, kThreads, p.lds, s>>>(g, bv, p.R, p.lpr, p.rpt, p.nc, ntiles, xx, yy, partials, counter,
+                                               dot_out, bv_L);
+}
+
+void launch_apply_r1(const LatticeConst* g, int L, BlockView bv, const double* x, double* y, int cus,
+                     double* partials, unsigned* counter, double* dot_out, hipStream_t s) {
+  if (bv.dim == 0) return;
+  int64_t grid = (bv.dim + kThreads - 1) / kThreads;
+  if (grid > (int64_t)cus * 8) grid = (int64_t)cus * 8;
+  k_apply_r1<<<(unsigned)grid, kThreads, table_lds_bytes(bv.nup, bv.ndn, L), s>>>(g, bv, reinterpret_cast<const double2*>(x),
+                                                                  reinterpret_cast<double2*>(y), partials, counter,
+                                                                  dot_out);
+}
+
+void launch_csr_count(const LatticeConst* g, BlockView bv, int all_dirs, int32_t* counts, hipStream_t s) {
+  if (bv.dim == 0) return;
+  k_csr_count<<<nblocks(bv.dim, 128), 128, 0, s>>>(g, bv, all_dirs, counts);
+}
+
+void launch_csr_fill(const LatticeConst* g, BlockView bv, int all_dirs, const int64_t* row_ptr, int32_t* col,
+                     double* val, double* diag, hipStream_t s) {
+  if (bv.dim == 0) return;
+  k_csr_fill<<<nblocks(bv.dim, 128), 128, 0, s>>>(g, bv, all_dirs, row_ptr, col, val, diag);
+}
+
+void launch_dense_build(const LatticeConst* g, const BlockView* bvs, int nblk, const int64_t* moff,
+                        double* mats, int64_t max_dim, hipStream_t s) {
+  if (nblk == 0 || max_dim == 0) return;
+  // moff[nblk] holds the total element count
+  int64_t total = 0;
+  (void)total;
+  dim3 grid(nblocks(max_dim, 64), nblk);
+  k_dense_rows<<<grid, 64, 0, s>>>(g, bvs, moff, mats, max_dim);
+  k_hermitise<<<grid, 64, 0, s>>>(bvs, moff, mats);
+}
+
+void launch_zero(double* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t grid = (n + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  k_zero<<<(unsigned)grid, 256, 0, s>>>(p, n);
+}
+
+}  // namespace ftlm

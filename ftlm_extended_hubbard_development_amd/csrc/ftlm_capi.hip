@@ -1,0 +1,1014 @@
+// ftlm_capi.hip -- C ABI of the FTLM engine (declared in include/ftlm.h).
+//
+// Host orchestration of the device kernels: sector and block tables,
+// Hamiltonian application, the batched Lanczos driver, the device Jacobi
+// tridiagonal solve and the batched exact eigensolver. Everything numeric
+// runs on the GPU except two inputs that must match the reference bit for
+// bit and are therefore produced by the same host library calls as there:
+// the twist / momentum phases (glibc cos, sin; main.cpp:544-577, 736-743)
+// and the seeded Gaussian start vectors (libstdc++ mt19937_64 +
+// normal_distribution; main.cpp:1015-1025).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ftlm.h"
+#include "ftlm_launch.h"
+
+using namespace ftlm;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                               \
+  do {                                                                                               \
+    const hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                            \
+      throw Error(e_ == hipErrorOutOfMemory ? FTLM_ERR_MEMORY : FTLM_ERR_DEVICE,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                                \
+  } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return FTLM_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return FTLM_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return FTLM_ERR_ARG;
+  }
+}
+
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void* get(size_t n) {
+    if (n == 0) n = 8;
+    if (n > bytes) {
+      if (p) HIPCHECK(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+      HIPCHECK(hipMalloc(&p, n));
+      bytes = n;
+    }
+    return p;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(get(count * sizeof(T)));
+  }
+  template <typename T>
+  T* ptr() const {
+    return static_cast<T*>(p);
+  }
+};
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+uint64_t binom_u64(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  if (k > n - k) k = n - k;
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+  if (r > (unsigned __int128)UINT64_MAX) return UINT64_MAX;
+  return (uint64_t)r;
+}
+
+// complex product with separately rounded products, like the reference
+inline void cmul_host(double ar, double ai, double br, double bi, double& cr, double& ci) {
+  cr = ar * br - ai * bi;
+  ci = ar * bi + ai * br;
+}
+
+// per-momentum tables of one block
+struct BlockTables {
+  int mx = -1, my = -1;
+  int64_t dim = 0;
+  DevBuf offs, nt_loc, nt_dnf, nt_list, nt_listoff, ptab, phase, act, counts, nt_count, tmp;
+  BlockView view{};
+};
+
+}  // namespace
+
+struct ftlm_ctx {
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  ftlm_model model{};
+  LatticeConst g{};
+  DevBuf g_dev, binom_dev;
+  std::vector<uint64_t> binom_host;
+  double amp_re[8], amp_im[8];
+  // sector
+  bool has_sector = false;
+  int nup = -1, ndn = -1;
+  int64_t n_upc = 0, n_dnc = 0;
+  int n_canon = 0, n_nt = 0;
+  int64_t parents = 0;
+  DevBuf is_canon, min_rank, upc_tfirst, upc_tsign, upc_amask, cidx, cnt, upc_canon, can_bits, can_stab, nt_flag,
+      nt_index, can_nt, nt_to_c;
+  // blocks
+  BlockTables cur;
+  std::vector<std::unique_ptr<BlockTables>> pool;
+  // Lanczos
+  DevBuf vec[3], part, counter, st_alpha, st_beta, st_scale, st_sprev, st_bprev, st_dot, st_nrm, st_done, st_mcount,
+      ritz_eig, ritz_w, ritz_scratch, pvec[3];
+  int prepared_R = 0;
+  int prepared_layout = -1;  // 0: [dim][R], 1: [R][dim]
+  ftlm_stats stats{};
+  // exact
+  DevBuf ex_mats, ex_work, ex_eig, ex_moff, ex_dims, ex_eoff, ex_views;
+
+  ~ftlm_ctx() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+void build_lattice(ftlm_ctx* c) {
+  const ftlm_model& m = c->model;
+  LatticeConst& g = c->g;
+  std::memset(&g, 0, sizeof(g));
+  g.lx = m.lx;
+  g.ly = m.ly;
+  g.L = m.lx * m.ly;
+  const int lx = m.lx, ly = m.ly;
+  auto at = [lx, ly](int x, int y) { return ((y % ly + ly) % ly) * lx + ((x % lx + lx) % lx); };
+  std::vector<std::pair<int, int>> bonds;
+  for (int y = 0; y < ly; ++y)
+    for (int x = 0; x < lx; ++x) {
+      const int s = at(x, y);
+      const int nbv[8] = {at(x + 1, y),     at(x - 1, y),     at(x, y + 1),     at(x, y - 1),
+                          at(x + 1, y + 1), at(x - 1, y + 1), at(x + 1, y - 1), at(x - 1, y - 1)};
+      for (int d = 0; d < 8; ++d) g.nb[s][d] = (uint8_t)nbv[d];
+      bonds.push_back({s, at(x + 1, y)});
+      bonds.push_back({s, at(x, y + 1)});
+    }
+  std::sort(bonds.begin(), bonds.end());
+  bonds.erase(std::unique(bonds.begin(), bonds.end()), bonds.end());
+  g.nbonds = (int)bonds.size();
+  for (size_t i = 0; i < bonds.size(); ++i) {
+    g.bonds[i][0] = (uint8_t)bonds[i].first;
+    g.bonds[i][1] = (uint8_t)bonds[i].second;
+  }
+  g.full = g.L == 64 ? ~0ull : ((1ull << g.L) - 1);
+  for (int k = 0; k <= lx; ++k) {
+    uint64_t mk = 0;
+    for (int y = 0; y < ly; ++y)
+      for (int x = k; x < lx; ++x) mk |= 1ull << (y * lx + x);
+    g.col_ge[k] = mk;
+  }
+  for (int k = 0; k <= ly; ++k) {
+    uint64_t mk = 0;
+    for (int y = 0; y < k; ++y)
+      for (int x = 0; x < lx; ++x) mk |= 1ull << (y * lx + x);
+    g.rows_lt[k] = mk;
+  }
+  g.U = m.u;
+  g.V = m.v;
+  // hopping directions with non-zero amplitude, and their amplitudes
+  // -t * twist (main.cpp:544-577, 683-687) for fermion sign +1
+  const double ax = 2.0 * kPi * m.phix / (double)lx;
+  const double ay = 2.0 * kPi * m.phiy / (double)ly;
+  const double ang[8] = {ax, -ax, ay, -ay, ax + ay, -ax + ay, ax - ay, -ax - ay};
+  g.ndir = 0;
+  for (int d = 0; d < 8; ++d) {
+    const double t = d < 2 ? m.tx : (d < 4 ? m.ty : m.tp);
+    const double a = -t * 1.0;
+    const double cr = std::cos(ang[d]), ci = std::sin(ang[d]);
+    c->amp_re[d] = a * cr;
+    c->amp_im[d] = a * ci;
+    if (t != 0.0) g.dirs[g.ndir++] = (int8_t)d;
+  }
+}
+
+void upload_ctx_tables(ftlm_ctx* c) {
+  HIPCHECK(hipMemcpy(c->g_dev.get(sizeof(LatticeConst)), &c->g, sizeof(LatticeConst), hipMemcpyHostToDevice));
+  c->binom_host.assign(64 * 64, 0);
+  for (int k = 0; k < 64; ++k)
+    for (int p = 0; p < 64; ++p) c->binom_host[k * 64 + p] = binom_u64(p, k);
+  HIPCHECK(hipMemcpy(c->binom_dev.get(64 * 64 * 8), c->binom_host.data(), 64 * 64 * 8, hipMemcpyHostToDevice));
+}
+
+const LatticeConst* gdev(ftlm_ctx* c) { return c->g_dev.ptr<LatticeConst>(); }
+
+void sync(ftlm_ctx* c) { HIPCHECK(hipStreamSynchronize(c->stream)); }
+
+template <typename T>
+T read1(ftlm_ctx* c, const void* dptr) {
+  T v;
+  HIPCHECK(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  return v;
+}
+
+void need_sector(ftlm_ctx* c) {
+  if (!c->has_sector) throw Error(FTLM_ERR_STATE, "no sector selected (call ftlm_sector first)");
+}
+void need_block(ftlm_ctx* c) {
+  need_sector(c);
+  if (c->cur.mx < 0) throw Error(FTLM_ERR_STATE, "no block selected (call ftlm_block first)");
+}
+
+void build_sector(ftlm_ctx* c, int nup, int ndn) {
+  const int L = c->g.L;
+  if (nup < 0 || ndn < 0 || nup > L || ndn > L) throw Error(FTLM_ERR_ARG, "particle numbers out of range");
+  const uint64_t nu = binom_u64(L, nup), nd = binom_u64(L, ndn);
+  if (nu > (uint64_t)INT32_MAX || nd > (uint64_t)INT32_MAX)
+    throw Error(FTLM_ERR_LIMIT, "spin configuration count exceeds 2^31");
+  c->has_sector = false;
+  c->cur.mx = c->cur.my = -1;
+  c->nup = nup;
+  c->ndn = ndn;
+  c->n_upc = (int64_t)nu;
+  c->n_dnc = (int64_t)nd;
+  hipStream_t s = c->stream;
+  const uint64_t* binom = c->binom_dev.ptr<uint64_t>();
+  uint8_t* is_canon = c->is_canon.as<uint8_t>(nu);
+  int64_t* min_rank = c->min_rank.as<int64_t>(nu);
+  uint8_t* tfirst = c->upc_tfirst.as<uint8_t>(nu);
+  int8_t* tsg = c->upc_tsign.as<int8_t>(nu);
+  uint64_t* amask = c->upc_amask.as<uint64_t>(nu);
+  int32_t* cidx = c->cidx.as<int32_t>(nu);
+  int64_t* cnt = c->cnt.as<int64_t>(4);
+  launch_up_table(gdev(c), binom, nup, (int64_t)nu, is_canon, min_rank, tfirst, tsg, amask, s);
+  launch_segscan(is_canon, (int64_t)nu, 1, cidx, cnt, s);
+  HIPCHECK(hipGetLastError());
+  c->n_canon = (int)read1<int64_t>(c, cnt);
+  const int nc = c->n_canon;
+  int32_t* upc_canon = c->upc_canon.as<int32_t>(nu);
+  uint64_t* can_bits = c->can_bits.as<uint64_t>(nc);
+  uint64_t* can_stab = c->can_stab.as<uint64_t>(nc);
+  uint8_t* nt_flag = c->nt_flag.as<uint8_t>(nc);
+  launch_canon_fill((int64_t)nu, nup, L, binom, is_canon, cidx, min_rank, amask, upc_canon, can_bits, can_stab,
+                    nt_flag, s);
+  int32_t* nt_index = c->nt_index.as<int32_t>(nc);
+  launch_segscan(nt_flag, nc, 1, nt_index, cnt, s);
+  HIPCHECK(hipGetLastError());
+  c->n_nt = (int)read1<int64_t>(c, cnt);
+  int32_t* can_nt = c->can_nt.as<int32_t>(nc);
+  int32_t* nt_to_c = c->nt_to_c.as<int32_t>(c->n_nt > 0 ? c->n_nt : 1);
+  launch_nt_fill(nc, nt_index, can_nt, nt_to_c, s);
+  HIPCHECK(hipGetLastError());
+  c->has_sector = true;
+}
+
+BlockView base_view(ftlm_ctx* c) {
+  BlockView v{};
+  v.nup = c->nup;
+  v.ndn = c->ndn;
+  v.n_canon = c->n_canon;
+  v.n_dnc = c->n_dnc;
+  v.upc_canon = c->upc_canon.ptr<int32_t>();
+  v.upc_tfirst = c->upc_tfirst.ptr<uint8_t>();
+  v.upc_tsign = c->upc_tsign.ptr<int8_t>();
+  v.upc_amask = c->upc_amask.ptr<uint64_t>();
+  v.can_bits = c->can_bits.ptr<uint64_t>();
+  v.can_stab = c->can_stab.ptr<uint64_t>();
+  v.can_nt = c->can_nt.ptr<int32_t>();
+  v.binom = c->binom_dev.ptr<uint64_t>();
+  return v;
+}
+
+// Per-momentum tables (main.cpp:736-805). Returns the block dimension.
+void build_block(ftlm_ctx* c, BlockTables& b, int mx, int my) {
+  const LatticeConst& g = c->g;
+  if (mx < 0 || my < 0 || mx >= g.lx || my >= g.ly) throw Error(FTLM_ERR_ARG, "momentum out of range");
+  hipStream_t s = c->stream;
+  const int L = g.L;
+  // momentum phases e^{i 2 pi (mx dx/lx + my dy/ly)} for shift j = dy*lx+dx
+  std::vector<double> ph(2 * L), pt(16 * L);
+  for (int j = 0; j < L; ++j) {
+    const int dx = j % g.lx, dy = j / g.lx;
+    const double arg =
+        2.0 * kPi * (static_cast<double>(mx * dx) / static_cast<double>(g.lx) +
+                     static_cast<double>(my * dy) / static_cast<double>(g.ly));
+    ph[2 * j] = std::cos(arg);
+    ph[2 * j + 1] = std::sin(arg);
+  }
+  for (int d = 0; d < 8; ++d)
+    for (int j = 0; j < L; ++j)
+      cmul_host(c->amp_re[d], c->amp_im[d], ph[2 * j], ph[2 * j + 1], pt[2 * (d * L + j)], pt[2 * (d * L + j) + 1]);
+  double* phd = b.phase.as<double>(2 * L);
+  double* ptd = b.ptab.as<double>(16 * L);
+  HIPCHECK(hipMemcpyAsync(phd, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(ptd, pt.data(), pt.size() * 8, hipMemcpyHostToDevice, s));
+  const int nnt = c->n_nt;
+  const int64_t ndc = c->n_dnc;
+  const int64_t ntot = (int64_t)nnt * ndc;
+  int32_t* nt_loc = b.nt_loc.as<int32_t>(ntot > 0 ? ntot : 1);
+  double* nt_dnf = b.nt_dnf.as<double>(ntot > 0 ? ntot : 1);
+  int64_t* nt_count = b.nt_count.as<int64_t>(nnt + 1);
+  int64_t* listoff = b.nt_listoff.as<int64_t>(nnt + 1);
+  int64_t list_total = 0;
+  if (nnt > 0) {
+    uint8_t* act = b.act.as<uint8_t>(ntot);
+    launch_nt_active(gdev(c), c->binom_dev.ptr<uint64_t>(), c->ndn, ndc, nnt, c->nt_to_c.ptr<int32_t>(),
+                     c->can_bits.ptr<uint64_t>(), c->can_stab.ptr<uint64_t>(), phd, act, nt_dnf, s);
+    launch_segscan(act, ndc, nnt, nt_loc, nt_count, s);
+    launch_prefix64(nt_count, nnt, listoff, s);
+    HIPCHECK(hipGetLastError());
+    list_total = read1<int64_t>(c, listoff + nnt);
+  }
+  int32_t* list = b.nt_list.as<int32_t>(list_total > 0 ? list_total : 1);
+  if (nnt > 0) launch_nt_list(nnt, ndc, nt_loc, listoff, list, s);
+  int64_t* counts = b.counts.as<int64_t>(c->n_canon);
+  int64_t* offs = b.offs.as<int64_t>(c->n_canon + 1);
+  launch_row_counts(c->n_canon, ndc, c->can_nt.ptr<int32_t>(), nt_count, counts, s);
+  launch_prefix64(counts, c->n_canon, offs, s);
+  HIPCHECK(hipGetLastError());
+  b.dim = read1<int64_t>(c, offs + c->n_canon);
+  b.mx = mx;
+  b.my = my;
+  BlockView v = base_view(c);
+  v.dim = b.dim;
+  v.offs = offs;
+  v.nt_loc = nt_loc;
+  v.nt_dnf = nt_dnf;
+  v.nt_list = list;
+  v.nt_listoff = listoff;
+  v.ptab = ptd;
+  b.view = v;
+}
+
+// Number of translation orbits (the reference's parents.size()):
+// trivial-stabiliser canonical ups own every down; an nt up owns the downs
+// that are minimal under its stabiliser. Counted on the host from the
+// device tables (only the few nt ups need work).
+int64_t count_parents(ftlm_ctx* c) {
+  if (c->n_nt == 0) return (int64_t)c->n_canon * c->n_dnc;
+  std::vector<uint64_t> cs(c->n_canon);
+  std::vector<int32_t> n2c(c->n_nt);
+  HIPCHECK(hipMemcpy(cs.data(), c->can_stab.p, 8 * c->n_canon, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(n2c.data(), c->nt_to_c.p, 4 * c->n_nt, hipMemcpyDeviceToHost));
+  const LatticeConst& g = c->g;
+  const int L = g.L;
+  auto tr = [&](uint64_t b, int i) {
+    const int dx = i % g.lx, dy = i / g.lx;
+    if (dx) b = ((b << dx) & g.col_ge[dx]) | ((b >> (g.lx - dx)) & ~g.col_ge[dx] & g.full);
+    if (dy) {
+      const int sft = g.lx * dy;
+      b = ((b << sft) | (b >> (L - sft))) & g.full;
+    }
+    return b;
+  };
+  std::vector<uint64_t> downs;
+  downs.reserve(c->n_dnc);
+  if (c->ndn == 0) {
+    downs.push_back(0);
+  } else {
+    uint64_t st = (1ull << c->ndn) - 1;
+    const uint64_t lim = 1ull << L;
+    while (st < lim) {
+      downs.push_back(st);
+      const uint64_t low = st & (~st + 1);
+      const uint64_t rip = st + low;
+      if (rip == 0) break;
+      st = (((rip ^ st) >> 2) / low) | rip;
+    }
+  }
+  int64_t reps = 0;
+  for (int x = 0; x < c->n_nt; ++x) {
+    const uint64_t stab = cs[n2c[x]] & ~1ull;
+    for (uint64_t d : downs) {
+      bool rep = true;
+      uint64_t m = stab;
+      while (m) {
+        const int i = __builtin_ctzll(m);
+        m &= m - 1;
+        if (tr(d, i) < d) {
+          rep = false;
+          break;
+        }
+      }
+      reps += rep;
+    }
+  }
+  return (int64_t)(c->n_canon - c->n_nt) * c->n_dnc + reps;
+}
+
+// ---- seeded start vectors (host; bit-identical to main.cpp:1015-1025) ----
+uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// one sample: Gaussian pairs with the real part taken from the SECOND draw
+// (g++ evaluates Complex(normal(rng), normal(rng)) right to left), then
+// division by the serially summed norm
+void gen_start(uint64_t bseed, int sample, int64_t dim, double* out) {
+  std::mt19937_64 rng(mix64(bseed ^ (uint64_t)sample));
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (int64_t i = 0; i < dim; ++i) {
+    const double first = nd(rng);
+    const double second = nd(rng);
+    out[2 * i] = second;
+    out[2 * i + 1] = first;
+  }
+  double s = 0.0;
+  for (int64_t i = 0; i < dim; ++i) s += out[2 * i] * out[2 * i] + out[2 * i + 1] * out[2 * i + 1];
+  const double nrm = std::sqrt(s);
+  if (nrm <= 0.0) throw Error(FTLM_ERR_ARG, "Attempted to normalize a zero vector.");
+  for (int64_t i = 0; i < 2 * dim; ++i) out[i] /= nrm;
+}
+
+int host_threads() {
+  const unsigned n = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(n, 32u));
+}
+
+// Generate samples r0..r0+cnt-1 into host memory (sample-major), threaded
+// over samples.
+void gen_starts(uint64_t bseed, int r0, int cnt, int64_t dim, double* host) {
+  const int nt = std::min(host_threads(), cnt);
+  std::vector<std::thread> th;
+  std::vector<std::string> errs(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      try {
+        for (int j = t; j < cnt; j += nt) gen_start(bseed, r0 + j, dim, host + 2 * (int64_t)j * dim);
+      } catch (const std::exception& e) {
+        errs[t] = e.what();
+      }
+    });
+  for (auto& x : th) x.join();
+  for (auto& e : errs)
+    if (!e.empty()) throw Error(FTLM_ERR_ARG, e);
+}
+
+// device vector buffers for R columns of the current block
+void ensure_vectors(ftlm_ctx* c, int R) {
+  const size_t bytes = (size_t)c->cur.dim * R * 16;
+  for (int i = 0; i < 3; ++i) c->vec[i].get(bytes);
+}
+
+// upload seeded start vectors of samples 0..R-1 into layout 0 ([dim][R] in
+// vec[0]) or layout 1 (sample-major [R][dim] in vec[0])
+void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout) {
+  const int64_t dim = c->cur.dim;
+  ensure_vectors(c, R);
+  const int chunk = std::max(1, std::min(R, host_threads()));
+  double* host = nullptr;
+  HIPCHECK(hipHostMalloc(&host, (size_t)chunk * dim * 16, hipHostMallocDefault));
+  std::unique_ptr<double, void (*)(double*)> guard(host, [](double* p) { (void)hipHostFree(p); });
+  double* x = c->vec[0].ptr<double>();
+  double* stage = c->vec[2].ptr<double>();
+  for (int r0 = 0; r0 < R; r0 += chunk) {
+    const int cnt = std::min(chunk, R - r0);
+    gen_starts(bseed, r0, cnt, dim, host);
+    if (layout == 1) {
+      HIPCHECK(hipMemcpyAsync(x + 2 * (int64_t)r0 * dim, host, (size_t)cnt * dim * 16, hipMemcpyHostToDevice,
+                              c->stream));
+    } else {
+      HIPCHECK(hipMemcpyAsync(stage, host, (size_t)cnt * dim * 16, hipMemcpyHostToDevice, c->stream));
+      launch_transpose_in(dim, R, r0, cnt, stage, x, c->stream);
+    }
+    sync(c);
+  }
+  c->prepared_R = R;
+  c->prepared_layout = layout;
+}
+
+LanczosState lanczos_state(ftlm_ctx* c, int R, int steps) {
+  LanczosState st;
+  st.alpha = c->st_alpha.as<double>((size_t)steps * R);
+  st.beta = c->st_beta.as<double>((size_t)steps * R);
+  st.scale = c->st_scale.as<double>(R);
+  st.sprev = c->st_sprev.as<double>(R);
+  st.bprev = c->st_bprev.as<double>(R);
+  st.dot = c->st_dot.as<double>(R);
+  st.nrm = c->st_nrm.as<double>(R);
+  st.done = c->st_done.as<int32_t>(R);
+  st.mcount = c->st_mcount.as<int32_t>(R);
+  return st;
+}
+
+// distinct non-zero-amplitude neighbour sites of one site (translation
+// invariant), for the algorithmic byte count
+double hbar(ftlm_ctx* c) {
+  const LatticeConst& g = c->g;
+  std::vector<int> sites;
+  for (int di = 0; di < g.ndir; ++di) {
+    const int t = g.nb[0][g.dirs[di]];
+    if (t != 0 && std::find(sites.begin(), sites.end(), t) == sites.end()) sites.push_back(t);
+  }
+  const double d = (double)sites.size();
+  const int L = g.L;
+  if (L < 2) return 0.0;
+  return d * ((double)c->nup * (L - c->nup) + (double)c->ndn * (L - c->ndn)) / (double)(L - 1);
+}
+
+// The Lanczos recurrence over R columns x (layout [dim][R]); x is the
+// normalised start in vec-buffer `xi`. Results stay on the device in the
+// LanczosState; `timed` records per-apply events into c->stats.
+void lanczos_run(ftlm_ctx* c, int R, int steps, int reduce, double* x0, double* x1, double* w, LanczosState st,
+                 bool r1_kernel, bool timed) {
+  const int64_t dim = c->cur.dim;
+  const int kdim = (int)std::min<int64_t>(steps, dim);
+  hipStream_t s = c->stream;
+  BlockView bv = c->cur.view;
+  ApplyPlan plan = plan_apply(c->g, c->nup, c->ndn, dim, R, c->cus);
+  if (r1_kernel && R != 1) throw Error(FTLM_ERR_ARG, "one-row kernel needs R = 1");
+  const int gridp = r1_kernel ? (int)std::min<int64_t>((dim + 255) / 256, (int64_t)c->cus * 8) : plan.grid;
+  double* partials = c->part.as<double>((size_t)std::max(gridp, 8192) * R + 64);
+  unsigned* counter = c->counter.as<unsigned>(4);
+  HIPCHECK(hipMemsetAsync(counter, 0, 16, s));
+  HIPCHECK(hipMemsetAsync(x1, 0, (size_t)dim * R * 16, s));
+  launch_lanczos_init(R, st, s);
+  const int rgrid = (int)std::max<int64_t>(1, std::min<int64_t>((dim * R + 255) / 256, (int64_t)c->cus * 8));
+  std::vector<hipEvent_t> ev;
+  if (timed) {
+    ev.resize(2 * kdim + 2);
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipEventRecord(ev[2 * kdim], s));
+  }
+  const bool tree = reduce == FTLM_REDUCE_TREE;
+  for (int k = 0; k < kdim; ++k) {
+    if (timed) HIPCHECK(hipEventRecord(ev[2 * k], s));
+    if (r1_kernel)
+      launch_apply_r1(gdev(c), c->g.L, bv, x0, w, c->cus, tree ? partials : nullptr, counter, st.dot, s);
+    else
+      launch_apply(plan, gdev(c), bv, x0, w, tree ? partials : nullptr, counter, st.dot, s);
+    if (timed) HIPCHECK(hipEventRecord(ev[2 * k + 1], s));
+    if (tree) {
+      launch_recur_tree(dim, R, w, x0, x1, k, kdim, st, partials, counter, rgrid, s);
+    } else {
+      launch_dot_serial(dim, R, x0, w, st.dot, s);
+      launch_recur_serial(dim, R, w, x0, x1, st.dot, st.bprev, st.done, s);
+      launch_norm_serial(dim, R, x1, st.nrm, s);
+      launch_finish_serial(R, k, kdim, st, s);
+      launch_normalize(dim, R, x1, st.beta + (size_t)k * R, st.done, k, st.mcount, s);
+    }
+    std::swap(x0, x1);
+  }
+  HIPCHECK(hipGetLastError());
+  if (timed) {
+    HIPCHECK(hipEventRecord(ev[2 * kdim + 1], s));
+    sync(c);
+    double tot = 0.0;
+    for (int k = 0; k < kdim; ++k) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      tot += ms;
+    }
+    float all = 0.f;
+    HIPCHECK(hipEventElapsedTime(&all, ev[2 * kdim], ev[2 * kdim + 1]));
+    c->stats.ms_apply += tot;
+    c->stats.ms_total += all;
+    c->stats.n_apply += kdim;
+    c->stats.bytes_apply += (double)kdim * (double)dim * R * 16.0 * (hbar(c) + 2.0);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+}
+
+// Lanczos of R columns stored in vec[0] (layout 0) or, per vector, of the
+// sample-major columns of vec[0] (layout 1). Leaves alpha/beta/mcount of all
+// R columns in the ctx state arrays ([steps][R]).
+void run_all(ftlm_ctx* c, int R, int steps, int reduce, bool per_vector, bool timed) {
+  const int64_t dim = c->cur.dim;
+  LanczosState st = lanczos_state(c, R, steps);
+  if (!per_vector) {
+    lanczos_run(c, R, steps, reduce, c->vec[0].ptr<double>(), c->vec[1].ptr<double>(), c->vec[2].ptr<double>(), st,
+                false, timed);
+    return;
+  }
+  // one column at a time through the R = 1 kernels
+  double* p0 = c->pvec[0].as<double>((size_t)dim * 2);
+  double* p1 = c->pvec[1].as<double>((size_t)dim * 2);
+  double* p2 = c->pvec[2].as<double>((size_t)dim * 2);
+  LanczosState one;
+  DevBuf a1, b1, sc1, sp1, bp1, d1, n1, dn1, m1;
+  one.alpha = a1.as<double>(steps);
+  one.beta = b1.as<double>(steps);
+  one.scale = sc1.as<double>(1);
+  one.sprev = sp1.as<double>(1);
+  one.bprev = bp1.as<double>(1);
+  one.dot = d1.as<double>(1);
+  one.nrm = n1.as<double>(1);
+  one.done = dn1.as<int32_t>(1);
+  one.mcount = m1.as<int32_t>(1);
+  for (int r = 0; r < R; ++r) {
+    HIPCHECK(hipMemcpyAsync(p0, c->vec[0].ptr<double>() + 2 * (int64_t)r * dim, (size_t)dim * 16,
+                            hipMemcpyDeviceToDevice, c->stream));
+    lanczos_run(c, 1, steps, reduce, p0, p1, p2, one, true, timed);
+    // scatter column r of alpha/beta/mcount
+    HIPCHECK(hipMemcpy2DAsync(st.alpha + r, R * 8, one.alpha, 8, 8, steps, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpy2DAsync(st.beta + r, R * 8, one.beta, 8, 8, steps, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(st.mcount + r, one.mcount, 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  sync(c);
+}
+
+void ritz_device(ftlm_ctx* c, int R, int steps, const double* alpha, const double* beta, const int32_t* mcount,
+                 int wdiv) {
+  double* eig = c->ritz_eig.as<double>((size_t)R * steps);
+  double* w = c->ritz_w.as<double>((size_t)R * steps);
+  const size_t words = (size_t)steps * (steps + 1) / 2 + 2 * (size_t)steps;
+  double* scratch = c->ritz_scratch.as<double>(words * R);
+  launch_ritz(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, c->stream);
+  HIPCHECK(hipGetLastError());
+}
+
+// copy Ritz values/weights of R columns back, concatenated in sample order
+int64_t gather_ritz(ftlm_ctx* c, int R, int steps, double* eig, double* w) {
+  std::vector<double> he((size_t)R * steps), hw((size_t)R * steps);
+  std::vector<int32_t> m(R);
+  HIPCHECK(hipMemcpyAsync(he.data(), c->ritz_eig.p, he.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(hw.data(), c->ritz_w.p, hw.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(m.data(), c->st_mcount.p, 4 * R, hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  int64_t n = 0;
+  for (int r = 0; r < R; ++r) {
+    for (int j = 0; j < m[r]; ++j) {
+      if (eig) eig[n] = he[(size_t)r * steps + j];
+      if (w) w[n] = hw[(size_t)r * steps + j];
+      ++n;
+    }
+  }
+  return n;
+}
+
+void check_R(int R) {
+  if (R < 1 || R > 1024) throw Error(FTLM_ERR_LIMIT, "R must be in [1, 1024]");
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* ftlm_last_error(void) { return g_err.c_str(); }
+
+uint64_t ftlm_block_seed(uint64_t seed, int nup, int ndn, int mx, int my) {
+  uint64_t s = mix64(seed);
+  s ^= mix64((uint64_t)(nup + 1));
+  s ^= mix64((uint64_t)(ndn + 17));
+  s ^= mix64((uint64_t)(mx + 257));
+  s ^= mix64((uint64_t)(my + 65537));
+  return mix64(s);
+}
+
+int ftlm_create(const ftlm_model* model, int device, ftlm_ctx** out) {
+  return guarded([&] {
+    if (!model || !out) throw Error(FTLM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (model->lx <= 0 || model->ly <= 0) throw Error(FTLM_ERR_ARG, "Lattice dimensions must be positive.");
+    if (model->lx * model->ly > 63) throw Error(FTLM_ERR_LIMIT, "This implementation supports at most 63 sites.");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+      throw Error(FTLM_ERR_DEVICE, "no HIP device available (the FTLM engine runs on the GPU only)");
+    if (device < 0 || device >= n) throw Error(FTLM_ERR_DEVICE, "device index out of range");
+    auto c = std::make_unique<ftlm_ctx>();
+    c->device = device;
+    HIPCHECK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->model = *model;
+    build_lattice(c.get());
+    upload_ctx_tables(c.get());
+    *out = c.release();
+  });
+}
+
+void ftlm_destroy(ftlm_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+int ftlm_sector(ftlm_ctx* ctx, int nup, int ndn, uint64_t* full_dim, int64_t* parents) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    build_sector(ctx, nup, ndn);
+    const uint64_t fd = binom_u64(ctx->g.L, nup) * binom_u64(ctx->g.L, ndn);
+    if (full_dim) *full_dim = fd;
+    if (parents) *parents = count_parents(ctx);
+  });
+}
+
+int ftlm_block(ftlm_ctx* ctx, int mx, int my, int64_t* dim) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_sector(ctx);
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->prepared_R = 0;
+    build_block(ctx, ctx->cur, mx, my);
+    if (dim) *dim = ctx->cur.dim;
+  });
+}
+
+int ftlm_block_reps(ftlm_ctx* ctx, uint64_t* up, uint64_t* dn, int32_t* deg, double* dnf) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    const int64_t n = ctx->cur.dim;
+    if (n == 0) return;
+    DevBuf a, b, d, f;
+    uint64_t* du = a.as<uint64_t>(n);
+    uint64_t* dd = b.as<uint64_t>(n);
+    int32_t* dg = d.as<int32_t>(n);
+    double* df = f.as<double>(n);
+    launch_block_reps(gdev(ctx), ctx->cur.view, du, dd, dg, df, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    if (up) HIPCHECK(hipMemcpyAsync(up, du, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (dn) HIPCHECK(hipMemcpyAsync(dn, dd, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (deg) HIPCHECK(hipMemcpyAsync(deg, dg, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (dnf) HIPCHECK(hipMemcpyAsync(dnf, df, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  });
+}
+
+int ftlm_block_csr(ftlm_ctx* ctx, int all_dirs, int64_t* nnz, int32_t* row_ptr, int32_t* col, double* val,
+                   double* diag) {
+  return guarded([&] {
+    if (!ctx || !nnz) throw Error(FTLM_ERR_ARG, "null argument");
+    need_block(ctx);
+    const int64_t n = ctx->cur.dim;
+    if (n == 0) {
+      *nnz = 0;
+      if (row_ptr) row_ptr[0] = 0;
+      return;
+    }
+    DevBuf cnt, rp, cl, vl, dg;
+    int32_t* dc = cnt.as<int32_t>(n);
+    launch_csr_count(gdev(ctx), ctx->cur.view, all_dirs, dc, ctx->stream);
+    std::vector<int32_t> hc(n);
+    HIPCHECK(hipMemcpyAsync(hc.data(), dc, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    std::vector<int64_t> hp(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) hp[i + 1] = hp[i] + hc[i];
+    *nnz = hp[n];
+    if (!col) return;
+    if (hp[n] > INT32_MAX) throw Error(FTLM_ERR_LIMIT, "nnz exceeds 2^31");
+    int64_t* drp = rp.as<int64_t>(n + 1);
+    HIPCHECK(hipMemcpyAsync(drp, hp.data(), (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    int32_t* dcol = cl.as<int32_t>(hp[n] + 1);
+    double* dval = vl.as<double>(2 * hp[n] + 2);
+    double* ddg = dg.as<double>(n);
+    launch_csr_fill(gdev(ctx), ctx->cur.view, all_dirs, drp, dcol, dval, ddg, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(col, dcol, hp[n] * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (val) HIPCHECK(hipMemcpyAsync(val, dval, hp[n] * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (diag) HIPCHECK(hipMemcpyAsync(diag, ddg, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    if (row_ptr)
+      for (int64_t i = 0; i <= n; ++i) row_ptr[i] = (int32_t)hp[i];
+  });
+}
+
+int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y) {
+  return guarded([&] {
+    if (!ctx || !x || !y) throw Error(FTLM_ERR_ARG, "null argument");
+    need_block(ctx);
+    check_R(R);
+    if (kernel == 1 && R != 1) throw Error(FTLM_ERR_ARG, "kernel 1 handles R = 1 only");
+    const int64_t n = ctx->cur.dim;
+    if (n == 0) return;
+    ensure_vectors(ctx, R);
+    double* dx = ctx->vec[0].ptr<double>();
+    double* dy = ctx->vec[1].ptr<double>();
+    HIPCHECK(hipMemcpyAsync(dx, x, (size_t)n * R * 16, hipMemcpyHostToDevice, ctx->stream));
+    if (kernel == 1) {
+      launch_apply_r1(gdev(ctx), ctx->g.L, ctx->cur.view, dx, dy, ctx->cus, nullptr, nullptr, nullptr, ctx->stream);
+    } else {
+      const ApplyPlan p = plan_apply(ctx->g, ctx->nup, ctx->ndn, n, R, ctx->cus);
+      launch_apply(p, gdev(ctx), ctx->cur.view, dx, dy, nullptr, nullptr, nullptr, ctx->stream);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(y, dy, (size_t)n * R * 16, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    ctx->prepared_R = 0;
+  });
+}
+
+int ftlm_start_vector(ftlm_ctx* ctx, uint64_t block_seed, int sample, double* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw Error(FTLM_ERR_ARG, "null argument");
+    need_block(ctx);
+    gen_start(block_seed, sample, ctx->cur.dim, out);
+  });
+}
+
+int ftlm_lanczos(ftlm_ctx* ctx, int R, int steps, const double* start, uint64_t block_seed, int reduce,
+                 int per_vector, double* alpha, double* beta, int32_t* m) {
+  return guarded([&] {
+    if (!ctx || !alpha || !beta || !m) throw Error(FTLM_ERR_ARG, "null argument");
+    need_block(ctx);
+    check_R(R);
+    if (steps < 1) throw Error(FTLM_ERR_ARG, "--samples and --lanczos-steps must be positive.");
+    const int64_t n = ctx->cur.dim;
+    if (n == 0) throw Error(FTLM_ERR_ARG, "empty block");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const int layout = per_vector ? 1 : 0;
+    if (start) {
+      ensure_vectors(ctx, R);
+      if (layout == 0) {
+        HIPCHECK(hipMemcpyAsync(ctx->vec[0].p, start, (size_t)n * R * 16, hipMemcpyHostToDevice, ctx->stream));
+      } else {
+        std::vector<double> tr((size_t)n * R * 2);
+        for (int r = 0; r < R; ++r)
+          for (int64_t i = 0; i < n; ++i) {
+            tr[2 * ((size_t)r * n + i)] = start[2 * ((size_t)i * R + r)];
+            tr[2 * ((size_t)r * n + i) + 1] = start[2 * ((size_t)i * R + r) + 1];
+          }
+        HIPCHECK(hipMemcpyAsync(ctx->vec[0].p, tr.data(), tr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+      }
+      sync(ctx);
+    } else {
+      prepare_starts(ctx, R, block_seed, layout);
+    }
+    run_all(ctx, R, steps, reduce, per_vector != 0, false);
+    HIPCHECK(hipMemcpyAsync(alpha, ctx->st_alpha.p, (size_t)steps * R * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(beta, ctx->st_beta.p, (size_t)steps * R * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(m, ctx->st_mcount.p, (size_t)R * 4, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    ctx->prepared_R = 0;
+  });
+}
+
+int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double* beta, const int32_t* m, int wdiv,
+              double* eig, double* w) {
+  return guarded([&] {
+    if (!ctx || !alpha || !beta || !m || !eig || !w) throw Error(FTLM_ERR_ARG, "null argument");
+    check_R(R);
+    DevBuf da, db, dm;
+    double* a = da.as<double>((size_t)steps * R);
+    double* b = db.as<double>((size_t)steps * R);
+    int32_t* mm = dm.as<int32_t>(R);
+    for (int r = 0; r < R; ++r)
+      if (m[r] < 0 || m[r] > steps) throw Error(FTLM_ERR_ARG, "m out of range");
+    HIPCHECK(hipMemcpyAsync(a, alpha, (size_t)steps * R * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(b, beta, (size_t)steps * R * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(mm, m, (size_t)R * 4, hipMemcpyHostToDevice, ctx->stream));
+    ritz_device(ctx, R, steps, a, b, mm, wdiv);
+    HIPCHECK(hipMemcpyAsync(eig, ctx->ritz_eig.p, (size_t)R * steps * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(w, ctx->ritz_w.p, (size_t)R * steps * 8, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  });
+}
+
+int ftlm_ftlm_block(ftlm_ctx* ctx, int R, int steps, uint64_t block_seed, int reduce, int per_vector, double* eig,
+                    double* w, int64_t* count) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    check_R(R);
+    if (steps < 1) throw Error(FTLM_ERR_ARG, "--samples and --lanczos-steps must be positive.");
+    if (ctx->cur.dim == 0) {
+      if (count) *count = 0;
+      return;
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    prepare_starts(ctx, R, block_seed, per_vector ? 1 : 0);
+    run_all(ctx, R, steps, reduce, per_vector != 0, false);
+    const int kdim = (int)std::min<int64_t>(steps, ctx->cur.dim);
+    ritz_device(ctx, R, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(), ctx->st_mcount.ptr<int32_t>(),
+                R);
+    // the state arrays are [steps][R]; the Ritz kernel reads rows of R, so
+    // using kdim rows is exact (rows >= kdim are unused)
+    const int64_t n = gather_ritz(ctx, R, kdim, eig, w);
+    if (count) *count = n;
+    ctx->prepared_R = 0;
+  });
+}
+
+int ftlm_prepare(ftlm_ctx* ctx, int R, uint64_t block_seed) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    check_R(R);
+    HIPCHECK(hipSetDevice(ctx->device));
+    prepare_starts(ctx, R, block_seed, 0);
+    // keep a pristine copy of the start vectors so that repeated timed runs
+    // start from the same input: vec[2] is the apply output and free here
+  });
+}
+
+int ftlm_run(ftlm_ctx* ctx, int R, int steps, int reduce, int per_vector_column) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    if (ctx->prepared_R != R || ctx->prepared_layout != 0) throw Error(FTLM_ERR_STATE, "call ftlm_prepare first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->stats = ftlm_stats{};
+    if (per_vector_column < 0) {
+      run_all(ctx, R, steps, reduce, false, true);
+    } else {
+      // plain per-vector mode on one column of the prepared block: extract
+      // it and run the R = 1 kernels
+      const int64_t dim = ctx->cur.dim;
+      if (per_vector_column >= R) throw Error(FTLM_ERR_ARG, "column out of range");
+      double* p0 = ctx->pvec[0].as<double>((size_t)dim * 2);
+      double* p1 = ctx->pvec[1].as<double>((size_t)dim * 2);
+      double* p2 = ctx->pvec[2].as<double>((size_t)dim * 2);
+      HIPCHECK(hipMemcpy2DAsync(p0, 16, ctx->vec[0].ptr<double>() + 2 * per_vector_column, (size_t)R * 16, 16, dim,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+      LanczosState one = lanczos_state(ctx, 1, steps);
+      lanczos_run(ctx, 1, steps, reduce, p0, p1, p2, one, true, true);
+    }
+  });
+}
+
+int ftlm_finish(ftlm_ctx* ctx, int R, int steps, double* eig, double* w, int64_t* count) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    const int kdim = (int)std::min<int64_t>(steps, ctx->cur.dim);
+    ritz_device(ctx, R, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(), ctx->st_mcount.ptr<int32_t>(),
+                R);
+    const int64_t n = gather_ritz(ctx, R, kdim, eig, w);
+    if (count) *count = n;
+  });
+}
+
+int ftlm_last_stats(ftlm_ctx* ctx, ftlm_stats* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw Error(FTLM_ERR_ARG, "null argument");
+    *out = ctx->stats;
+  });
+}
+
+int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t* my, double* eig_out,
+                      int64_t* offsets) {
+  return guarded([&] {
+    if (!ctx || (nblk > 0 && (!mx || !my || !eig_out || !offsets))) throw Error(FTLM_ERR_ARG, "null argument");
+    need_sector(ctx);
+    HIPCHECK(hipSetDevice(ctx->device));
+    if (nblk <= 0) {
+      if (offsets) offsets[0] = 0;
+      return;
+    }
+    while ((int)ctx->pool.size() < nblk) ctx->pool.emplace_back(new BlockTables);
+    std::vector<BlockView> views(nblk);
+    std::vector<int64_t> moff(nblk + 1, 0), dims(nblk), eoff(nblk + 1, 0);
+    int64_t maxd = 0;
+    for (int i = 0; i < nblk; ++i) {
+      build_block(ctx, *ctx->pool[i], mx[i], my[i]);
+      views[i] = ctx->pool[i]->view;
+      dims[i] = ctx->pool[i]->dim;
+      if (dims[i] > 8192) throw Error(FTLM_ERR_LIMIT, "exact diagonalisation is limited to dim <= 8192");
+      moff[i + 1] = moff[i] + dims[i] * dims[i];
+      eoff[i + 1] = eoff[i] + dims[i];
+      maxd = std::max(maxd, dims[i]);
+    }
+    hipStream_t s = ctx->stream;
+    double* mats = ctx->ex_mats.as<double>(2 * (size_t)moff[nblk] + 2);
+    double* work = ctx->ex_work.as<double>(6 * (size_t)eoff[nblk] + 6);
+    double* eig = ctx->ex_eig.as<double>((size_t)eoff[nblk] + 1);
+    int64_t* dmoff = ctx->ex_moff.as<int64_t>(nblk + 1);
+    int64_t* ddims = ctx->ex_dims.as<int64_t>(nblk);
+    int64_t* deoff = ctx->ex_eoff.as<int64_t>(nblk + 1);
+    BlockView* dviews = ctx->ex_views.as<BlockView>(nblk);
+    HIPCHECK(hipMemcpyAsync(dmoff, moff.data(), (nblk + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(ddims, dims.data(), nblk * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(deoff, eoff.data(), (nblk + 1) * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dviews, views.data(), nblk * sizeof(BlockView), hipMemcpyHostToDevice, s));
+    launch_zero(mats, 2 * moff[nblk], s);
+    launch_dense_build(gdev(ctx), dviews, nblk, dmoff, mats, maxd, s);
+    launch_herm_eig(dmoff, ddims, nblk, mats, work, eig, deoff, s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(eig_out, eig, eoff[nblk] * 8, hipMemcpyDeviceToHost, s));
+    sync(ctx);
+    for (int i = 0; i <= nblk; ++i) offsets[i] = eoff[i];
+    for (int64_t i = 0; i < eoff[nblk]; ++i)
+      if (std::isnan(eig_out[i])) throw Error(FTLM_ERR_DEVICE, "exact eigensolver did not converge");
+  });
+}
+
+}  // extern "C"

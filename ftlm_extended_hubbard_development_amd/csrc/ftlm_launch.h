@@ -1,0 +1,94 @@
+// ftlm_launch.h -- host-side launchers of the FTLM kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ftlm_types.h"
+
+namespace ftlm {
+
+// ---- basis (ftlm_basis.hip) ----
+// Per up configuration: orbit minimum, shifts reaching it, their sign.
+void launch_up_table(const LatticeConst* g, const uint64_t* binom, int nup, int64_t n_upc, uint8_t* is_canon,
+                     int64_t* min_rank, uint8_t* tfirst, int8_t* tsign, uint64_t* amask, hipStream_t s);
+// Segmented exclusive compaction: out[s][i] = flag ? rank among set flags of
+// segment s : -1 ; counts[s] = number of set flags. One workgroup/segment.
+void launch_segscan(const uint8_t* flags, int64_t seg_len, int nseg, int32_t* out, int64_t* counts,
+                    hipStream_t s);
+// Exclusive prefix sum of n int64 values into out[0..n] (out[n] = total).
+void launch_prefix64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
+void launch_canon_fill(int64_t n_upc, int nup, int L, const uint64_t* binom, const uint8_t* is_canon,
+                       const int32_t* cidx_of_rank, const int64_t* min_rank, const uint64_t* amask,
+                       int32_t* upc_canon, uint64_t* can_bits, uint64_t* can_stab, uint8_t* nt_flag,
+                       hipStream_t s);
+void launch_nt_fill(int n_canon, const int32_t* nt_index, int32_t* can_nt, int32_t* nt_to_c, hipStream_t s);
+void launch_nt_active(const LatticeConst* g, const uint64_t* binom, int ndn, int64_t n_dnc, int n_nt,
+                      const int32_t* nt_to_c, const uint64_t* can_bits, const uint64_t* can_stab,
+                      const double* phase_k, uint8_t* act, double* dnf, hipStream_t s);
+void launch_nt_list(int n_nt, int64_t n_dnc, const int32_t* nt_loc, const int64_t* listoff, int32_t* list,
+                    hipStream_t s);
+void launch_row_counts(int n_canon, int64_t n_dnc, const int32_t* can_nt, const int64_t* nt_count,
+                       int64_t* counts, hipStream_t s);
+// Rows -> (up, down, degeneracy, norm factor) for export.
+void launch_block_reps(const LatticeConst* g, BlockView bv, uint64_t* up, uint64_t* dn, int32_t* deg,
+                       double* dnf, hipStream_t s);
+
+// ---- H apply (ftlm_apply.hip) ----
+// y = H x over R column vectors stored row-major [dim][R] (complex double).
+// When dot != nullptr also accumulates per-column Re<x, y> partial sums into
+// dot[grid][R] and reduces them in the last workgroup into dot_out[R].
+struct ApplyPlan {
+  int R;
+  int lpr;      // lanes per row
+  int vpl;      // vectors per lane
+  int rpt;      // rows per tile
+  int nc;       // hop candidate slots per row
+  int grid;     // workgroups (persistent)
+  int L;        // sites
+  size_t lds;   // dynamic LDS bytes
+};
+ApplyPlan plan_apply(const LatticeConst& g, int nup, int ndn, int64_t dim, int R, int cus);
+void launch_apply(const ApplyPlan& p, const LatticeConst* g, BlockView bv, const double* x, double* y,
+                  double* partials, unsigned* counter, double* dot_out, hipStream_t s);
+void launch_apply_r1(const LatticeConst* g, int L, BlockView bv, const double* x, double* y, int cus,
+                     double* partials, unsigned* counter, double* dot_out, hipStream_t s);
+void launch_zero(double* p, int64_t n, hipStream_t s);
+// CSR export: counts per row (pass 1) and entries (pass 2).
+void launch_csr_count(const LatticeConst* g, BlockView bv, int all_dirs, int32_t* counts, hipStream_t s);
+void launch_csr_fill(const LatticeConst* g, BlockView bv, int all_dirs, const int64_t* row_ptr, int32_t* col,
+                     double* val, double* diag, hipStream_t s);
+// Dense column-major Hermitised matrices of a batch of small blocks.
+void launch_dense_build(const LatticeConst* g, const BlockView* bvs, int nblk, const int64_t* moff,
+                        double* mats, int64_t max_dim, hipStream_t s);
+
+// ---- Lanczos (ftlm_lanczos.hip) ----
+struct LanczosState {
+  double* alpha;   // [steps][R]
+  double* beta;    // [steps][R]
+  double* scale;   // [R] 1/b of the current vector (tree mode)
+  double* sprev;   // [R]
+  double* bprev;   // [R]
+  double* dot;     // [R] reduction output
+  double* nrm;     // [R] reduction output
+  int32_t* done;   // [R]
+  int32_t* mcount; // [R]
+};
+void launch_lanczos_init(int R, LanczosState st, hipStream_t s);
+void launch_dot_serial(int64_t dim, int R, const double* x, const double* y, double* out, hipStream_t s);
+void launch_norm_serial(int64_t dim, int R, const double* x, double* out, hipStream_t s);
+void launch_recur_serial(int64_t dim, int R, const double* w, const double* q, double* qp, const double* alpha_k,
+                         const double* bprev, const int32_t* done, hipStream_t s);
+void launch_finish_serial(int R, int k, int kdim, LanczosState st, hipStream_t s);
+void launch_normalize(int64_t dim, int R, double* q, const double* beta_k, const int32_t* done, int k,
+                      const int32_t* mcount, hipStream_t s);
+void launch_recur_tree(int64_t dim, int R, const double* y, const double* x, double* xp, int k, int kdim,
+                       LanczosState st, double* partials, unsigned* counter, int grid, hipStream_t s);
+void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src, double* x, hipStream_t s);
+void launch_ritz(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
+                 double* eig, double* w, double* scratch, hipStream_t s);
+
+// ---- exact (ftlm_exact.hip) ----
+void launch_herm_eig(const int64_t* moff, const int64_t* dims, int nblk, double* mats, double* work,
+                     double* eig_out, const int64_t* eoff, hipStream_t s);
+
+}  // namespace ftlm

@@ -1,0 +1,132 @@
+/* ftlm.h -- C ABI of the MI355X FTLM engine (libftlm_hip.so).
+ *
+ * The reference (PeizhiMai/FTLM_extended_Hubbard_development) has no
+ * library or FFI: its hot path is three functions of one translation unit,
+ * called from the block loop of src/main.cpp:1519-1728. This header is the
+ * boundary that replaces them; the drop-in executables ftlm_n_vs_mu and
+ * ftlm_ed_n_vs_mu (host C++) call only what is declared here.
+ *
+ *   build_sector_basis   main.cpp:622-721  -> ftlm_sector
+ *   build_momentum_block main.cpp:723-805  -> ftlm_block (+ exports)
+ *   apply_hamiltonian    main.cpp:833-845  -> ftlm_apply
+ *   run_ftlm_block       main.cpp:996-1113 -> ftlm_ftlm_block (ftlm_lanczos)
+ *   jacobi_diagonalize   main.cpp:913-994  -> ftlm_ritz (device Jacobi)
+ *   diagonalize_block_exact main.cpp:847-911 -> ftlm_exact_blocks
+ *   make_block_seed      main.cpp:520-533  -> ftlm_block_seed
+ *
+ * Conventions: every call returns 0 (FTLM_OK) or a negative FTLM_ERR_*
+ * code, with a message from ftlm_last_error(); calls are synchronous at
+ * return; host buffers belong to the caller; a context is used from one host
+ * thread at a time. Complex vectors are interleaved (re, im) doubles; R
+ * vectors of a block are stored row-major [dim][R], i.e. element (i, r) at
+ * 2*(i*R + r).
+ */
+#ifndef FTLM_H
+#define FTLM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTLM_OK 0
+#define FTLM_ERR_ARG (-1)     /* invalid argument */
+#define FTLM_ERR_DEVICE (-2)  /* HIP runtime / no usable GPU */
+#define FTLM_ERR_MEMORY (-3)  /* device or host allocation failed */
+#define FTLM_ERR_STATE (-4)   /* call order (no sector / block selected) */
+#define FTLM_ERR_LIMIT (-5)   /* size beyond what the engine supports */
+
+/* reduction orders of the Lanczos dot products and norms */
+#define FTLM_REDUCE_SERIAL 0  /* row order, bit-identical to the reference */
+#define FTLM_REDUCE_TREE 1    /* fused workgroup tree, deterministic */
+
+typedef struct ftlm_ctx ftlm_ctx;
+
+/* Model of Params (main.cpp:32-61): lattice and Hamiltonian parameters. */
+typedef struct ftlm_model {
+  int32_t lx, ly;
+  double tx, ty, tp, phix, phiy, u, v;
+} ftlm_model;
+
+/* Device-event timings of the last Lanczos run on a context. */
+typedef struct ftlm_stats {
+  double ms_apply;     /* summed H-apply kernel time */
+  double ms_total;     /* first to last kernel of the run */
+  int64_t n_apply;     /* H-apply launches */
+  double bytes_apply;  /* algorithmic bytes of those launches */
+} ftlm_stats;
+
+const char* ftlm_last_error(void);
+
+/* SplitMix64-based block seed (main.cpp:514-533). */
+uint64_t ftlm_block_seed(uint64_t seed, int nup, int ndn, int mx, int my);
+
+/* Create a context on HIP device `device` (fails with FTLM_ERR_DEVICE when
+ * no GPU is usable: there is no CPU fallback). */
+int ftlm_create(const ftlm_model* model, int device, ftlm_ctx** out);
+void ftlm_destroy(ftlm_ctx* ctx);
+
+/* Select the (N_up, N_down) sector and build its tables on the device.
+ * full_dim = C(L,Nup) C(L,Ndn); parents = number of translation orbits
+ * (the reference's SectorBasis::parents.size()). Either out may be NULL. */
+int ftlm_sector(ftlm_ctx* ctx, int nup, int ndn, uint64_t* full_dim, int64_t* parents);
+
+/* Select momentum block (mx, my) of the current sector; dim = active rows. */
+int ftlm_block(ftlm_ctx* ctx, int mx, int my, int64_t* dim);
+
+/* Representatives (up, down bitstrings), degeneracy and norm factor of each
+ * row of the current block, in the reference's row order. */
+int ftlm_block_reps(ftlm_ctx* ctx, uint64_t* up, uint64_t* dn, int32_t* deg, double* dnf);
+
+/* CSR of the current block as the reference stores it (row = source state,
+ * entries in hop order). all_dirs != 0 keeps zero-amplitude directions like
+ * the reference; 0 drops them. Call with col == NULL to get nnz only. */
+int ftlm_block_csr(ftlm_ctx* ctx, int all_dirs, int64_t* nnz, int32_t* row_ptr, int32_t* col, double* val,
+                   double* diag);
+
+/* y = H x for R vectors of the current block (host buffers, 2*dim*R each).
+ * kernel: 0 = batched block-vector kernel, 1 = one-row-per-thread kernel. */
+int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y);
+
+/* Seeded unit-norm complex Gaussian start vector of one sample
+ * (main.cpp:1015-1025), 2*dim doubles. */
+int ftlm_start_vector(ftlm_ctx* ctx, uint64_t block_seed, int sample, double* out);
+
+/* Lanczos coefficients of R vectors on the current block.
+ * start: host [dim][R] complex, or NULL for the seeded vectors of samples
+ * 0..R-1. per_vector != 0 runs each vector alone (R = 1 kernels).
+ * alpha, beta: [steps][R] (beta has m-1 valid rows per column); m: [R]. */
+int ftlm_lanczos(ftlm_ctx* ctx, int R, int steps, const double* start, uint64_t block_seed, int reduce,
+                 int per_vector, double* alpha, double* beta, int32_t* m);
+
+/* Ritz values (ascending) and weights v1^2 / wdiv of R tridiagonal matrices
+ * with the reference's Jacobi method, on the device. alpha/beta: [steps][R]. */
+int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double* beta, const int32_t* m,
+              int wdiv, double* eig, double* w);
+
+/* run_ftlm_block: R seeded samples, `steps` Lanczos steps, Ritz values and
+ * weights concatenated in sample order (eig/w hold R*min(steps, dim)). */
+int ftlm_ftlm_block(ftlm_ctx* ctx, int R, int steps, uint64_t block_seed, int reduce, int per_vector,
+                    double* eig, double* w, int64_t* count);
+
+/* Benchmark split of ftlm_ftlm_block: upload the seeded start vectors, then
+ * run the Lanczos recurrence on resident data (timed with device events, see
+ * ftlm_last_stats), then solve the tridiagonal matrices. */
+int ftlm_prepare(ftlm_ctx* ctx, int R, uint64_t block_seed);
+int ftlm_run(ftlm_ctx* ctx, int R, int steps, int reduce, int per_vector_column);
+int ftlm_finish(ftlm_ctx* ctx, int R, int steps, double* eig, double* w, int64_t* count);
+int ftlm_last_stats(ftlm_ctx* ctx, ftlm_stats* out);
+
+/* diagonalize_block_exact for blocks (mx[i], my[i]) of the current sector,
+ * batched in one launch: eig_out receives the ascending eigenvalues of each
+ * block back to back, offsets[i]..offsets[i+1]. */
+int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t* my, double* eig_out,
+                      int64_t* offsets);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTLM_H */
