@@ -54,7 +54,7 @@ SIGNATURES = {
     "ftlm_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
     "ftlm_start_vector": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, _dp]),
     "ftlm_lanczos": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, C.c_uint64, C.c_int, C.c_int, _dp, _dp, _ip]),
-    "ftlm_ritz": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp]),
+    "ftlm_ritz": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _ip, C.c_int, C.c_int, _dp, _dp]),
     "ftlm_ftlm_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, _dp, _dp, _lp]),
     "ftlm_prepare": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
     "ftlm_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -185,14 +185,14 @@ class Engine:
                                           _p(b, _dp), _p(m, _ip)))
         return a, b, m
 
-    def ritz(self, alpha, beta, m, wdiv):
+    def ritz(self, alpha, beta, m, wdiv, method=0):
         alpha = np.ascontiguousarray(alpha, np.float64)
         beta = np.ascontiguousarray(beta, np.float64)
         m = np.ascontiguousarray(m, np.int32)
         steps, R = alpha.shape
         e = np.zeros((R, steps))
         w = np.zeros((R, steps))
-        self._check(self.lib.ftlm_ritz(self.h, R, steps, _p(alpha, _dp), _p(beta, _dp), _p(m, _ip), wdiv,
+        self._check(self.lib.ftlm_ritz(self.h, R, steps, _p(alpha, _dp), _p(beta, _dp), _p(m, _ip), wdiv, method,
                                        _p(e, _dp), _p(w, _dp)))
         return e, w
 

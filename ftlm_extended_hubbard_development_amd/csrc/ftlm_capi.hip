@@ -14,6 +14,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -141,7 +142,8 @@ struct ftlm_ctx {
   std::vector<std::unique_ptr<BlockTables>> pool;
   // Lanczos
   DevBuf vec[3], part, counter, st_alpha, st_beta, st_scale, st_sprev, st_bprev, st_dot, st_nrm, st_done, st_mcount,
-      ritz_eig, ritz_w, ritz_scratch, pvec[3];
+      ritz_eig, ritz_w, ritz_scratch, ritz_iscratch, ritz_fail, pvec[3], pristine;
+  int last_reduce = FTLM_REDUCE_TREE;
   int prepared_R = 0;
   int prepared_layout = -1;  // 0: [dim][R], 1: [R][dim]
   ftlm_stats stats{};
@@ -444,9 +446,17 @@ void gen_start(uint64_t bseed, int sample, int64_t dim, double* out) {
   for (int64_t i = 0; i < 2 * dim; ++i) out[i] /= nrm;
 }
 
+// Host threads for start-vector generation: FTLM_HOST_THREADS, else
+// OMP_NUM_THREADS, else the hardware count, capped at 64.
 int host_threads() {
+  for (const char* var : {"FTLM_HOST_THREADS", "OMP_NUM_THREADS"}) {
+    if (const char* e = std::getenv(var)) {
+      const int n = std::atoi(e);
+      if (n > 0) return std::min(n, 64);
+    }
+  }
   const unsigned n = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(n, 32u));
+  return (int)std::max(1u, std::min(n, 64u));
 }
 
 // Generate samples r0..r0+cnt-1 into host memory (sample-major), threaded
@@ -479,7 +489,9 @@ void ensure_vectors(ftlm_ctx* c, int R) {
 void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout) {
   const int64_t dim = c->cur.dim;
   ensure_vectors(c, R);
-  const int chunk = std::max(1, std::min(R, host_threads()));
+  // pinned staging of at most ~2 GB (or one sample)
+  const int64_t fit = std::max<int64_t>(1, (int64_t)(2ll << 30) / std::max<int64_t>(1, dim * 16));
+  const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)R, (int64_t)host_threads(), fit}));
   double* host = nullptr;
   HIPCHECK(hipHostMalloc(&host, (size_t)chunk * dim * 16, hipHostMallocDefault));
   std::unique_ptr<double, void (*)(double*)> guard(host, [](double* p) { (void)hipHostFree(p); });
@@ -631,13 +643,26 @@ void run_all(ftlm_ctx* c, int R, int steps, int reduce, bool per_vector, bool ti
   sync(c);
 }
 
+// method 0: the reference's Jacobi (bit-identical); 1: implicit QL
 void ritz_device(ftlm_ctx* c, int R, int steps, const double* alpha, const double* beta, const int32_t* mcount,
-                 int wdiv) {
+                 int wdiv, int method) {
   double* eig = c->ritz_eig.as<double>((size_t)R * steps);
   double* w = c->ritz_w.as<double>((size_t)R * steps);
-  const size_t words = (size_t)steps * (steps + 1) / 2 + 2 * (size_t)steps;
-  double* scratch = c->ritz_scratch.as<double>(words * R);
-  launch_ritz(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, c->stream);
+  if (method == 0) {
+    const size_t words = (size_t)steps * (steps + 1) / 2 + 3 * (size_t)steps;
+    double* scratch = c->ritz_scratch.as<double>(words * R);
+    launch_ritz(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, c->stream);
+  } else {
+    double* scratch = c->ritz_scratch.as<double>((size_t)3 * steps * R);
+    int32_t* isc = c->ritz_iscratch.as<int32_t>((size_t)steps * R + 1);
+    int32_t* fail = c->ritz_fail.as<int32_t>(1);
+    HIPCHECK(hipMemsetAsync(fail, 0, 4, c->stream));
+    launch_ritz_ql(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, isc, fail, c->stream);
+    int32_t nf = 0;
+    HIPCHECK(hipMemcpyAsync(&nf, fail, 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    if (nf) throw Error(FTLM_ERR_DEVICE, "tridiagonal QL did not converge");
+  }
   HIPCHECK(hipGetLastError());
 }
 
@@ -863,7 +888,7 @@ int ftlm_lanczos(ftlm_ctx* ctx, int R, int steps, const double* start, uint64_t 
 }
 
 int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double* beta, const int32_t* m, int wdiv,
-              double* eig, double* w) {
+              int method, double* eig, double* w) {
   return guarded([&] {
     if (!ctx || !alpha || !beta || !m || !eig || !w) throw Error(FTLM_ERR_ARG, "null argument");
     check_R(R);
@@ -876,7 +901,7 @@ int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double
     HIPCHECK(hipMemcpyAsync(a, alpha, (size_t)steps * R * 8, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpyAsync(b, beta, (size_t)steps * R * 8, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipMemcpyAsync(mm, m, (size_t)R * 4, hipMemcpyHostToDevice, ctx->stream));
-    ritz_device(ctx, R, steps, a, b, mm, wdiv);
+    ritz_device(ctx, R, steps, a, b, mm, wdiv, method);
     HIPCHECK(hipMemcpyAsync(eig, ctx->ritz_eig.p, (size_t)R * steps * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipMemcpyAsync(w, ctx->ritz_w.p, (size_t)R * steps * 8, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
@@ -898,10 +923,10 @@ int ftlm_ftlm_block(ftlm_ctx* ctx, int R, int steps, uint64_t block_seed, int re
     prepare_starts(ctx, R, block_seed, per_vector ? 1 : 0);
     run_all(ctx, R, steps, reduce, per_vector != 0, false);
     const int kdim = (int)std::min<int64_t>(steps, ctx->cur.dim);
+    // serial runs keep the reference's Jacobi (bit-identical Ritz pairs);
+    // tree runs use QL. The state arrays are [steps][R]; rows >= kdim unused.
     ritz_device(ctx, R, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(), ctx->st_mcount.ptr<int32_t>(),
-                R);
-    // the state arrays are [steps][R]; the Ritz kernel reads rows of R, so
-    // using kdim rows is exact (rows >= kdim are unused)
+                R, reduce == FTLM_REDUCE_SERIAL ? 0 : 1);
     const int64_t n = gather_ritz(ctx, R, kdim, eig, w);
     if (count) *count = n;
     ctx->prepared_R = 0;
@@ -915,8 +940,11 @@ int ftlm_prepare(ftlm_ctx* ctx, int R, uint64_t block_seed) {
     check_R(R);
     HIPCHECK(hipSetDevice(ctx->device));
     prepare_starts(ctx, R, block_seed, 0);
-    // keep a pristine copy of the start vectors so that repeated timed runs
-    // start from the same input: vec[2] is the apply output and free here
+    // keep the start vectors resident so that every timed run starts from
+    // the same input
+    const size_t bytes = (size_t)ctx->cur.dim * R * 16;
+    HIPCHECK(hipMemcpyAsync(ctx->pristine.get(bytes), ctx->vec[0].p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    sync(ctx);
   });
 }
 
@@ -927,7 +955,10 @@ int ftlm_run(ftlm_ctx* ctx, int R, int steps, int reduce, int per_vector_column)
     if (ctx->prepared_R != R || ctx->prepared_layout != 0) throw Error(FTLM_ERR_STATE, "call ftlm_prepare first");
     HIPCHECK(hipSetDevice(ctx->device));
     ctx->stats = ftlm_stats{};
+    ctx->last_reduce = reduce;
+    const size_t bytes = (size_t)ctx->cur.dim * R * 16;
     if (per_vector_column < 0) {
+      HIPCHECK(hipMemcpyAsync(ctx->vec[0].p, ctx->pristine.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
       run_all(ctx, R, steps, reduce, false, true);
     } else {
       // plain per-vector mode on one column of the prepared block: extract
@@ -937,7 +968,7 @@ int ftlm_run(ftlm_ctx* ctx, int R, int steps, int reduce, int per_vector_column)
       double* p0 = ctx->pvec[0].as<double>((size_t)dim * 2);
       double* p1 = ctx->pvec[1].as<double>((size_t)dim * 2);
       double* p2 = ctx->pvec[2].as<double>((size_t)dim * 2);
-      HIPCHECK(hipMemcpy2DAsync(p0, 16, ctx->vec[0].ptr<double>() + 2 * per_vector_column, (size_t)R * 16, 16, dim,
+      HIPCHECK(hipMemcpy2DAsync(p0, 16, ctx->pristine.ptr<double>() + 2 * per_vector_column, (size_t)R * 16, 16, dim,
                                 hipMemcpyDeviceToDevice, ctx->stream));
       LanczosState one = lanczos_state(ctx, 1, steps);
       lanczos_run(ctx, 1, steps, reduce, p0, p1, p2, one, true, true);
@@ -951,7 +982,7 @@ int ftlm_finish(ftlm_ctx* ctx, int R, int steps, double* eig, double* w, int64_t
     need_block(ctx);
     const int kdim = (int)std::min<int64_t>(steps, ctx->cur.dim);
     ritz_device(ctx, R, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(), ctx->st_mcount.ptr<int32_t>(),
-                R);
+                R, ctx->last_reduce == FTLM_REDUCE_SERIAL ? 0 : 1);
     const int64_t n = gather_ritz(ctx, R, kdim, eig, w);
     if (count) *count = n;
   });

@@ -231,6 +231,150 @@ __device__ __forceinline__ int64_t pk(int i, int j, int n) {
   return (int64_t)i * n - ((int64_t)i * (i - 1)) / 2 + (j - i);
 }
 
+// ---- ordering of the Ritz values --------------------------------------
+// The reference orders each sample's Ritz values with std::sort over an
+// index array (main.cpp:1076-1081). std::sort is not stable, so with
+// exactly tied eigenvalues (converged ghost copies) the weights' order
+// depends on the algorithm. To keep records byte-identical, the same
+// introsort is run here on one lane: median-of-three quicksort partitions
+// down to runs of 16, a heapsort fallback after 2*floor(log2 n) levels, then
+// a final insertion sort.
+struct ByValue {
+  const double* v;
+  __device__ bool operator()(int a, int b) const { return v[a] < v[b]; }
+};
+
+__device__ void lin_insert(int* a, int last, const ByValue& lt) {
+  const int val = a[last];
+  int next = last - 1;
+  while (lt(val, a[next])) {
+    a[last] = a[next];
+    last = next;
+    --next;
+  }
+  a[last] = val;
+}
+
+__device__ void insertion_sort(int* a, int lo, int hi, const ByValue& lt) {
+  if (lo == hi) return;
+  for (int i = lo + 1; i < hi; ++i) {
+    if (lt(a[i], a[lo])) {
+      const int val = a[i];
+      for (int j = i; j > lo; --j) a[j] = a[j - 1];
+      a[lo] = val;
+    } else {
+      lin_insert(a, i, lt);
+    }
+  }
+}
+
+__device__ void sift_down(int* a, int first, int hole, int len, int value, const ByValue& lt) {
+  const int top = hole;
+  int child = hole;
+  while (child < (len - 1) / 2) {
+    child = 2 * (child + 1);
+    if (lt(a[first + child], a[first + child - 1])) --child;
+    a[first + hole] = a[first + child];
+    hole = child;
+  }
+  if ((len & 1) == 0 && child == (len - 2) / 2) {
+    child = 2 * (child + 1);
+    a[first + hole] = a[first + child - 1];
+    hole = child - 1;
+  }
+  int parent = (hole - 1) / 2;
+  while (hole > top && lt(a[first + parent], value)) {
+    a[first + hole] = a[first + parent];
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  a[first + hole] = value;
+}
+
+__device__ void heap_sort_range(int* a, int first, int last, const ByValue& lt) {
+  const int len = last - first;
+  if (len >= 2) {
+    for (int parent = (len - 2) / 2;; --parent) {
+      sift_down(a, first, parent, len, a[first + parent], lt);
+      if (parent == 0) break;
+    }
+  }
+  while (last - first > 1) {
+    --last;
+    const int value = a[last];
+    a[last] = a[first];
+    sift_down(a, first, 0, last - first, value, lt);
+  }
+}
+
+__device__ void swap_at(int* a, int i, int j) {
+  const int t = a[i];
+  a[i] = a[j];
+  a[j] = t;
+}
+
+__device__ int partition_pivot(int* a, int first, int last, const ByValue& lt) {
+  const int mid = first + (last - first) / 2;
+  const int x = first + 1, y = mid, z = last - 1;
+  if (lt(a[x], a[y])) {
+    if (lt(a[y], a[z])) swap_at(a, first, y);
+    else if (lt(a[x], a[z])) swap_at(a, first, z);
+    else swap_at(a, first, x);
+  } else if (lt(a[x], a[z])) {
+    swap_at(a, first, x);
+  } else if (lt(a[y], a[z])) {
+    swap_at(a, first, z);
+  } else {
+    swap_at(a, first, y);
+  }
+  int lo = first + 1, hi = last;
+  while (true) {
+    while (lt(a[lo], a[first])) ++lo;
+    --hi;
+    while (lt(a[first], a[hi])) --hi;
+    if (!(lo < hi)) return lo;
+    swap_at(a, lo, hi);
+    ++lo;
+  }
+}
+
+__device__ void intro_sort(int* a, int n, const ByValue& lt) {
+  if (n <= 1) return;
+  int lg = 0;
+  while ((1 << (lg + 1)) <= n) ++lg;
+  // explicit stack of (first, last, depth); sub-ranges are disjoint, so the
+  // order in which they are finished does not change the result
+  int sf[64], sl[64], sd[64];
+  int top = 0;
+  sf[0] = 0;
+  sl[0] = n;
+  sd[0] = 2 * lg;
+  ++top;
+  while (top > 0) {
+    --top;
+    int first = sf[top], last = sl[top], depth = sd[top];
+    while (last - first > 16) {
+      if (depth == 0) {
+        heap_sort_range(a, first, last, lt);
+        break;
+      }
+      --depth;
+      const int cut = partition_pivot(a, first, last, lt);
+      sf[top] = cut;
+      sl[top] = last;
+      sd[top] = depth;
+      ++top;
+      last = cut;
+    }
+  }
+  if (n > 16) {
+    insertion_sort(a, 0, 16, lt);
+    for (int i = 16; i < n; ++i) lin_insert(a, i, lt);
+  } else {
+    insertion_sort(a, 0, n, lt);
+  }
+}
+
 __global__ void __launch_bounds__(64) k_ritz(int R, int steps, const double* __restrict__ alpha,
                                              const double* __restrict__ beta, const int32_t* __restrict__ mcount,
                                              int wdiv, double* __restrict__ eig_out, double* __restrict__ w_out,
@@ -240,7 +384,7 @@ __global__ void __launch_bounds__(64) k_ritz(int R, int steps, const double* __r
   const int lane = threadIdx.x;
   const int n = mcount[r];
   const int64_t np = (int64_t)steps * (steps + 1) / 2;
-  double* A = use_lds ? lds : scratch + (int64_t)r * (np + 2 * steps);
+  double* A = use_lds ? lds : scratch + (int64_t)r * (np + 3 * steps);
   double* v0 = A + np;
   double* ev = v0 + steps;
   const int64_t nn = (int64_t)n * (n + 1) / 2;
@@ -321,18 +465,106 @@ __global__ void __launch_bounds__(64) k_ritz(int R, int steps, const double* __r
       __syncthreads();
     }
   }
-  // ascending order, ties by index (stable)
-  for (int i = lane; i < n; i += 64) ev[i] = A[pk(i, i, n)];
-  __syncthreads();
+  int* order = reinterpret_cast<int*>(ev + steps);
   for (int i = lane; i < n; i += 64) {
-    const double e = ev[i];
-    int pos = 0;
-    for (int j = 0; j < n; ++j) {
-      const double f = ev[j];
-      pos += (f < e) || (f == e && j < i);
-    }
-    eig_out[(int64_t)r * steps + pos] = e;
-    w_out[(int64_t)r * steps + pos] = (v0[i] * v0[i]) / (double)wdiv;
+    ev[i] = A[pk(i, i, n)];
+    order[i] = i;
+  }
+  __syncthreads();
+  if (lane == 0) intro_sort(order, n, ByValue{ev});
+  __syncthreads();
+  for (int k = lane; k < n; k += 64) {
+    const int o = order[k];
+    eig_out[(int64_t)r * steps + k] = ev[o];
+    w_out[(int64_t)r * steps + k] = (v0[o] * v0[o]) / (double)wdiv;
+  }
+}
+
+// ---- implicit QL on the Lanczos tridiagonal (tree-order runs) ------------
+// One thread per column: symmetric tridiagonal QL with Wilkinson shifts
+// (the EISPACK tql2 scheme the reference's Fortran ancestor uses), carrying
+// only the first row of the eigenvector matrix, which is all the weights
+// need. O(m^2) per column instead of the Jacobi sweep's O(m^4); eigenvalues
+// agree with Jacobi's to rounding. Ordering uses the same introsort.
+__device__ __forceinline__ double hyp(double a, double b) {
+  a = fabs(a);
+  b = fabs(b);
+  const double hi = a > b ? a : b, lo = a > b ? b : a;
+  if (hi == 0.0) return 0.0;
+  const double r = lo / hi;
+  return hi * sqrt(1.0 + r * r);
+}
+
+__global__ void k_ritz_ql(int R, int steps, const double* __restrict__ alpha, const double* __restrict__ beta,
+                          const int32_t* __restrict__ mcount, int wdiv, double* __restrict__ eig_out,
+                          double* __restrict__ w_out, double* __restrict__ scratch, int32_t* __restrict__ iscratch,
+                          int32_t* __restrict__ fail) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int n = mcount[r];
+  double* d = scratch + (int64_t)r * 3 * steps;
+  double* e = d + steps;
+  double* z = e + steps;
+  int* order = iscratch + (int64_t)r * steps;
+  for (int i = 0; i < n; ++i) {
+    d[i] = alpha[(int64_t)i * R + r];
+    e[i] = i + 1 < n ? beta[(int64_t)i * R + r] : 0.0;
+    z[i] = i == 0 ? 1.0 : 0.0;
+  }
+  bool ok = true;
+  for (int l = 0; l < n && ok; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 60) {
+          ok = false;
+          break;
+        }
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double rr = hyp(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? rr : -rr));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          const double f = s * e[i];
+          const double b = c * e[i];
+          rr = hyp(f, g);
+          e[i + 1] = rr;
+          if (rr == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / rr;
+          c = g / rr;
+          g = d[i + 1] - p;
+          rr = (d[i] - g) * s + 2.0 * c * b;
+          p = s * rr;
+          d[i + 1] = g + p;
+          g = c * rr - b;
+          const double zf = z[i + 1];
+          z[i + 1] = s * z[i] + c * zf;
+          z[i] = c * z[i] - s * zf;
+        }
+        if (rr == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  if (!ok) atomicAdd(fail, 1);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  intro_sort(order, n, ByValue{d});
+  for (int k = 0; k < n; ++k) {
+    const int o = order[k];
+    eig_out[(int64_t)r * steps + k] = d[o];
+    w_out[(int64_t)r * steps + k] = (z[o] * z[o]) / (double)wdiv;
   }
 }
 
@@ -391,9 +623,14 @@ void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src,
                                       reinterpret_cast<double2*>(x));
 }
 
+void launch_ritz_ql(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
+                    double* eig, double* w, double* scratch, int32_t* iscratch, int32_t* fail, hipStream_t s) {
+  k_ritz_ql<<<nblocks(R, 64), 64, 0, s>>>(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, iscratch, fail);
+}
+
 void launch_ritz(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
                  double* eig, double* w, double* scratch, hipStream_t s) {
-  const size_t words = (size_t)steps * (steps + 1) / 2 + 2 * (size_t)steps;
+  const size_t words = (size_t)steps * (steps + 1) / 2 + 3 * (size_t)steps;
   const size_t bytes = words * 8;
   const int use_lds = bytes <= 60 * 1024;
   k_ritz<<<R, 64, use_lds ? bytes : 0, s>>>(R, steps, alpha, beta, mcount, wdiv, eig, w, scratch, use_lds);

@@ -84,6 +84,8 @@ void launch_normalize(int64_t dim, int R, double* q, const double* beta_k, const
 void launch_recur_tree(int64_t dim, int R, const double* y, const double* x, double* xp, int k, int kdim,
                        LanczosState st, double* partials, unsigned* counter, int grid, hipStream_t s);
 void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src, double* x, hipStream_t s);
+void launch_ritz_ql(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
+                    double* eig, double* w, double* scratch, int32_t* iscratch, int32_t* fail, hipStream_t s);
 void launch_ritz(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
                  double* eig, double* w, double* scratch, hipStream_t s);
 

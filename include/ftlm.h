@@ -102,9 +102,12 @@ int ftlm_lanczos(ftlm_ctx* ctx, int R, int steps, const double* start, uint64_t 
                  int per_vector, double* alpha, double* beta, int32_t* m);
 
 /* Ritz values (ascending) and weights v1^2 / wdiv of R tridiagonal matrices
- * with the reference's Jacobi method, on the device. alpha/beta: [steps][R]. */
+ * on the device. method 0: the reference's Jacobi, bit-identical
+ * (jacobi_diagonalize + std::sort ordering); method 1: implicit QL carrying
+ * the first eigenvector row (used by tree-order runs). alpha/beta:
+ * [steps][R]; eig, w: [R][steps]. */
 int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double* beta, const int32_t* m,
-              int wdiv, double* eig, double* w);
+              int wdiv, int method, double* eig, double* w);
 
 /* run_ftlm_block: R seeded samples, `steps` Lanczos steps, Ritz values and
  * weights concatenated in sample order (eig/w hold R*min(steps, dim)). */

@@ -152,11 +152,33 @@ def test_ritz_jacobi_bitexact(pkg, orc, n):
     al = rng.standard_normal((n, R)) * 3.0
     be = np.abs(rng.standard_normal((n, R))) + 0.1
     m = np.full(R, n, np.int32)
-    e, w = eng.ritz(al, be, m, 7)
+    e, w = eng.ritz(al, be, m, 7, method=0)
     for r in range(R):
         oe, ow = orc.ritz(al[:, r], be[: n - 1, r], 7)
         assert np.array_equal(e[r, :n], oe)
         assert np.array_equal(w[r, :n], ow)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 40, 80, 120])
+def test_ritz_ql_vs_numpy(pkg, n):
+    """QL Ritz pairs (tree-order runs) vs numpy eigh of the tridiagonal:
+    eigenvalues to 1e-13 * ||T||, weights summed over each cluster of equal
+    eigenvalues to 1e-12 (individual weights of degenerate pairs are not
+    unique)."""
+    eng = engine(pkg, "2x2")
+    rng = np.random.default_rng(100 + n)
+    R = 4
+    al = rng.standard_normal((n, R)) * 3.0
+    be = np.abs(rng.standard_normal((n, R))) + 0.1
+    m = np.full(R, n, np.int32)
+    e, w = eng.ritz(al, be, m, 1, method=1)
+    for r in range(R):
+        T = np.diag(al[:, r]) + np.diag(be[: n - 1, r], 1) + np.diag(be[: n - 1, r], -1)
+        ev, V = np.linalg.eigh(T)
+        scale = np.abs(ev).max()
+        assert np.max(np.abs(e[r, :n] - ev)) < 1e-13 * scale
+        assert np.max(np.abs(w[r, :n] - V[0] ** 2)) < 1e-12
+        assert abs(w[r, :n].sum() - 1.0) < 1e-13
 
 
 @pytest.mark.parametrize("case,R,steps", [(("4x2", 4, 4, 0, 0), 32, 120), (("3x3tw", 4, 3, 1, 2), 8, 80),
