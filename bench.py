@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--block", default="8,8,0,0")
     ap.add_argument("--plain-vectors", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", default="auto",
+                    help="H-apply configuration: auto (autotuner) or PATH,W (path 1 hop list / 2 on the fly; "
+                         "W -1 automatic, 0 untiled, >0 tile width)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine runs each FTLM block on a single device "
@@ -108,6 +111,13 @@ def main():
     t0 = time.perf_counter()
     eng.prepare(R, bseed)
     print(f"# start vectors resident ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    if args.tile == "auto":
+        tile, tile_ms, tune = eng.autotune(R)
+        print(f"# autotune: {tune} -> {tile}", file=sys.stderr, flush=True)
+    else:
+        path, w = (int(t) for t in args.tile.split(","))
+        eng.set_apply_config(path, w)
+        tile, tune = eng.apply_config(R), None
 
     for w in range(args.warmup):
         eng.run(R, m, ftlm.REDUCE_TREE)
@@ -158,7 +168,9 @@ def main():
         "data": "synthetic: seeded unit-norm complex Gaussian start vectors (reference stream, seed 12345)",
         "config": {"workload": f"4x4 U=7 FTLM block (Nup={nup},Ndown={ndn},k=({mx},{my})) dim={dim}, "
                                f"R={R} batched vectors, {m} Lanczos steps, tree reductions",
-                   "lattice": "4x4", "u": 7.0, "R": R, "lanczos_steps": m, "block_dim": dim},
+                   "lattice": "4x4", "u": 7.0, "R": R, "lanczos_steps": m, "block_dim": dim,
+                   "apply_config": {"path": ["auto", "hoplist", "onthefly"][tile[0]], "tile": tile[1]},
+                   "autotune_ms_per_apply": tune},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                      "algorithmic_bytes_per_launch": per_launch,

@@ -61,6 +61,9 @@ SIGNATURES = {
     "ftlm_finish": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _lp]),
     "ftlm_last_stats": (C.c_int, [C.c_void_p, C.POINTER(FtlmStats)]),
     "ftlm_exact_blocks": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _lp]),
+    "ftlm_set_apply_config": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "ftlm_get_apply_config": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
+    "ftlm_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
 }
 
 _lib = None
@@ -161,7 +164,8 @@ class Engine:
         return rp, col[: nnz.value], val[: nnz.value], diag[:n]
 
     def apply(self, x, kernel=0):
-        """x: complex (dim,) or (dim, R) -> H x (same shape)."""
+        """x: complex (dim,) or (dim, R) -> H x (same shape); kernel 0 uses
+        the apply configuration, 1 the on-the-fly one-row kernel (R = 1)."""
         x2 = np.ascontiguousarray(np.asarray(x).reshape(self.dim, -1), dtype=np.complex128)
         y = np.zeros_like(x2)
         self._check(self.lib.ftlm_apply(self.h, x2.shape[1], kernel, _p(x2, _dp), _p(y, _dp)))
@@ -222,6 +226,33 @@ class Engine:
         cnt = C.c_int64(0)
         self._check(self.lib.ftlm_finish(self.h, R, steps, _p(e, _dp), _p(w, _dp), C.byref(cnt)))
         return e[: cnt.value], w[: cnt.value]
+
+    def set_apply_config(self, path=0, W=-1):
+        """path: 0 automatic, 1 hop list, 2 on the fly; W: -1 automatic, 0
+        untiled, >0 column tile width."""
+        self._check(self.lib.ftlm_set_apply_config(self.h, path, W))
+
+    def apply_config(self, R):
+        p = C.c_int32(0)
+        w = C.c_int32(0)
+        self._check(self.lib.ftlm_get_apply_config(self.h, R, C.byref(p), C.byref(w)))
+        return p.value, w.value
+
+    def autotune(self, R):
+        """Time the apply candidates on the current block; returns
+        ((path, W), best ms, {"path/W": ms})."""
+        bp = C.c_int32(0)
+        bw = C.c_int32(0)
+        bms = C.c_double(0)
+        n = C.c_int32(16)
+        cp = np.zeros(16, np.int32)
+        cw = np.zeros(16, np.int32)
+        cm = np.zeros(16)
+        self._check(self.lib.ftlm_autotune(self.h, R, C.byref(bp), C.byref(bw), C.byref(bms), C.byref(n),
+                                           _p(cp, _ip), _p(cw, _ip), _p(cm, _dp)))
+        names = {1: "hoplist", 2: "onthefly"}
+        return ((bp.value, bw.value), bms.value,
+                {f"{names[int(cp[i])]}/W{int(cw[i])}": round(float(cm[i]), 3) for i in range(n.value)})
 
     def exact_blocks(self, ks):
         """ks: list of (mx, my) of the current sector -> list of eigenvalue arrays."""

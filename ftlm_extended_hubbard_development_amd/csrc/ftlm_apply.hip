@@ -21,6 +21,8 @@
 // Tiles are split into 8 contiguous ranges, one per XCD (workgroup g runs on
 // XCD g % 8), so rows that share an up configuration -- whose down-spin hops
 // land within the same few MB -- are processed under one L2.
+#include <vector>
+
 #include "ftlm_device.h"
 #include "ftlm_launch.h"
 
@@ -240,6 +242,207 @@ __global__ void __launch_bounds__(kThreads) k_apply(const LatticeConst* __restri
   if (tid == 0) *counter = 0u;
 }
 
+// Column-tiled batched kernel (k_apply_tiled). The R columns are cut into
+// tiles of W; work items are (canonical up group c, column tile j) in that
+// order and each XCD walks a contiguous range of them, so at any time the
+// workgroups of one XCD share one item. Down-spin hops of a row stay inside
+// its group, so their gathers hit the group's dim_c x W x 16 B slab, which is
+// sized to stay in the XCD's L2; only up-spin hops, the row itself and the
+// output go to HBM. Hop metadata is recomputed per tile (ALU is cheap here).
+template <int VPL>
+__global__ void __launch_bounds__(kThreads) k_apply_tiled(const LatticeConst* __restrict__ gp, BlockView bv, int R,
+                                                          int W, int lpr, int rpt, int nc, int J,
+                                                          const int64_t* __restrict__ item_tiles, int n_items,
+                                                          int64_t ntiles, const double2* __restrict__ x,
+                                                          double2* __restrict__ y, double* __restrict__ partials,
+                                                          unsigned* counter, double* __restrict__ dot_out, int L) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ LatticeConst sg;
+  const int kmax = (bv.nup > bv.ndn ? bv.nup : bv.ndn) + 1;
+  uint64_t* sbinom = reinterpret_cast<uint64_t*>(smem);
+  double* sptab = reinterpret_cast<double*>(sbinom + (kmax + 1) * 64);
+  double2* scoef = reinterpret_cast<double2*>(reinterpret_cast<uint64_t*>(smem) + table_words(kmax, L));
+  int32_t* scol = reinterpret_cast<int32_t*>(scoef + (size_t)rpt * nc);
+  double* sdiag = reinterpret_cast<double*>(scol + (((size_t)rpt * nc + 1) & ~(size_t)1));
+  double* sdnf = sdiag + rpt;
+  uint64_t* sup = reinterpret_cast<uint64_t*>(sdnf + rpt);
+  uint64_t* sdn = sup + rpt;
+  double* tpart = reinterpret_cast<double*>(sdn + rpt);  // [rpt][W]
+  double* acc_r = tpart + (size_t)rpt * W;              // [R]
+  stage_tables(gp, bv, kmax, sg, sbinom, sptab);
+  BlockView lv = bv;
+  lv.binom = sbinom;
+  lv.ptab = sptab;
+  const int tid = threadIdx.x;
+  const bool dot = partials != nullptr;
+  if (dot)
+    for (int r = tid; r < R; r += kThreads) acc_r[r] = 0.0;
+  const int ne = bv.nup + bv.ndn;
+  const int ndir = sg.ndir;
+  const int rl = tid / lpr;
+  const int lane = tid % lpr;
+
+  int64_t t1 = ntiles, tstep = gridDim.x, tfirst = blockIdx.x;
+  if (gridDim.x >= kXcd) {
+    const int xcd = blockIdx.x % kXcd;
+    const int l = blockIdx.x / kXcd;
+    const int nwg = (gridDim.x - xcd + kXcd - 1) / kXcd;
+    const int64_t chunk = (ntiles + kXcd - 1) / kXcd;
+    const int64_t t0 = xcd * chunk;
+    t1 = t0 + chunk < ntiles ? t0 + chunk : ntiles;
+    tfirst = t0 + l;
+    tstep = nwg;
+  }
+  for (int64_t tile = tfirst; tile < t1; tile += tstep) {
+    // item = last it with item_tiles[it] <= tile
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (item_tiles[mid] <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    const int c = lo / J, j = lo - (lo / J) * J;
+    const int64_t row0 = bv.offs[c] + (tile - item_tiles[lo]) * rpt;
+    const int64_t rend = bv.offs[c + 1];
+    const int cb = j * W;
+    const int wj = (cb + W <= R) ? W : R - cb;
+    if (tid < rpt) {
+      const int64_t row = row0 + tid;
+      if (row < rend) {
+        const RowState st = row_state_in(lv, sg, c, row);
+        sup[tid] = st.up;
+        sdn[tid] = st.dn;
+        sdnf[tid] = st.dnf;
+        sdiag[tid] = diag_energy(st.up, st.dn, sg);
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < rpt * ne; t += kThreads) {
+      const int r = t / ne, e = t - r * ne;
+      if (row0 + r >= rend) continue;
+      const uint64_t up = sup[r], dn = sdn[r];
+      const int spin = e >= bv.nup;
+      const uint64_t bits = spin ? dn : up;
+      const int from = select_bit(bits, spin ? e - bv.nup : e);
+      for (int di = 0; di < ndir; ++di) {
+        const int dir = sg.dirs[di];
+        const int to = sg.nb[from][dir];
+        const int slot = r * nc + e * ndir + di;
+        int64_t col = -1;
+        cd cf = {0.0, 0.0};
+        if (!((bits >> to) & 1ull)) {
+          if (!hop_target(lv, sg, c, up, dn, sdnf[r], spin, from, to, dir, &col, &cf)) col = -1;
+        }
+        scol[slot] = (int32_t)col;
+        scoef[slot] = make_double2(cf.re, cf.im);
+      }
+    }
+    __syncthreads();
+    double part[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) part[v] = 0.0;
+    if (rl < rpt) {
+      const int64_t row = row0 + rl;
+      if (row < rend) {
+        const double dg = sdiag[rl];
+        cd acc[VPL], xs[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int cc = lane + v * lpr;
+          xs[v] = {0.0, 0.0};
+          if (cc < wj) {
+            const double2 t = x[row * R + cb + cc];
+            xs[v] = {t.x, t.y};
+          }
+          acc[v] = {0.0 + dg * xs[v].re, 0.0 + dg * xs[v].im};
+        }
+        const int32_t* mycol = scol + rl * nc;
+        const double2* mycf = scoef + rl * nc;
+        int s = 0;
+        for (; s + 4 <= nc; s += 4) {
+          int32_t cl[4];
+          double2 cf[4];
+          double2 xv[4][VPL];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cl[u] = mycol[s + u];
+            cf[u] = mycf[s + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+              const int cc = lane + v * lpr;
+              xv[u][v] = (cl[u] >= 0 && cc < wj) ? x[(int64_t)cl[u] * R + cb + cc] : make_double2(0.0, 0.0);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (cl[u] < 0) continue;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v)
+              acc[v] = cadd(acc[v], cmul({cf[u].x, cf[u].y}, {xv[u][v].x, xv[u][v].y}));
+          }
+        }
+        for (; s < nc; ++s) {
+          const int32_t cl = mycol[s];
+          if (cl < 0) continue;
+          const double2 cf = mycf[s];
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            const int cc = lane + v * lpr;
+            if (cc < wj) {
+              const double2 t = x[(int64_t)cl * R + cb + cc];
+              acc[v] = cadd(acc[v], cmul({cf.x, cf.y}, {t.x, t.y}));
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int cc = lane + v * lpr;
+          if (cc < wj) {
+            y[row * R + cb + cc] = make_double2(acc[v].re, acc[v].im);
+            part[v] = xs[v].re * acc[v].re + xs[v].im * acc[v].im;
+          }
+        }
+      }
+    }
+    if (dot) {
+      // per-tile column sums in row order, added to the workgroup's column
+      // accumulators in tile order (deterministic)
+      if (rl < rpt) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int cc = lane + v * lpr;
+          if (cc < W) tpart[rl * W + cc] = part[v];
+        }
+      }
+      __syncthreads();
+      for (int cc = tid; cc < wj; cc += kThreads) {
+        double sum = 0.0;
+        for (int q = 0; q < rpt; ++q) sum += tpart[q * W + cc];
+        acc_r[cb + cc] += sum;
+      }
+    }
+    __syncthreads();
+  }
+  if (!dot) return;
+  for (int r = tid; r < R; r += kThreads) partials[(int64_t)blockIdx.x * R + r] = acc_r[r];
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (tid == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = tid; r < R; r += kThreads) {
+    double sum = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) sum += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
+    dot_out[r] = sum;
+  }
+  if (tid == 0) *counter = 0u;
+}
+
 // One row per thread: the per-vector (R = 1) Hamiltonian application.
 __global__ void __launch_bounds__(kThreads) k_apply_r1(const LatticeConst* __restrict__ gp, BlockView bv,
                                                        const double2* __restrict__ x, double2* __restrict__ y,
@@ -291,6 +494,287 @@ __global__ void __launch_bounds__(kThreads) k_apply_r1(const LatticeConst* __res
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int i = 0; i < kThreads; ++i) s += red[i];
+    partials[blockIdx.x] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[b]);
+    dot_out[0] = s;
+    *counter = 0u;
+  }
+}
+
+// ---- hop lists: the block's hops built once, reused by every apply -------
+// Entry = (col int32, code uint32): bit 31 = negate, bit 30 = "special",
+// bits 0..29 = index into ptab ([dir][shift], coefficient amp*phase) or, for
+// special hops (norm ratio != 1, i.e. a non-trivial stabiliser on either
+// end), into a per-block array holding the full coefficient. Hops are stored
+// in the reference's order and zero-amplitude directions are left out.
+constexpr uint32_t kNeg = 1u << 31, kSpecial = 1u << 30;
+
+// pass 1: hops and special hops per row
+__global__ void k_hops_count(const LatticeConst* __restrict__ gp, BlockView bv, int64_t* cnt, int64_t* scnt) {
+  const LatticeConst& g = *gp;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= bv.dim) return;
+  const RowState st = row_state(bv, g, row);
+  int n = 0, ns = 0;
+  for (int spin = 0; spin < 2; ++spin) {
+    uint64_t occ = spin ? st.dn : st.up;
+    const uint64_t bits = occ;
+    while (occ) {
+      const int from = ctz(occ);
+      occ &= occ - 1;
+      for (int di = 0; di < g.ndir; ++di) {
+        const int dir = g.dirs[di];
+        const int to = g.nb[from][dir];
+        if ((bits >> to) & 1ull) continue;
+        int64_t col;
+        cd cf;
+        bool special = false;
+        if (!hop_target_ex(bv, g, st.c, st.up, st.dn, st.dnf, spin, from, to, dir, &col, &cf, nullptr, &special))
+          continue;
+        ++n;
+        ns += special;
+      }
+    }
+  }
+  cnt[row] = n;
+  scnt[row] = ns;
+}
+
+// pass 2: fill (row_ptr, srow_ptr are exclusive prefix sums of the counts)
+__global__ void k_hops_fill(const LatticeConst* __restrict__ gp, BlockView bv, const int64_t* __restrict__ row_ptr,
+                            const int64_t* __restrict__ srow_ptr, int32_t* __restrict__ hcol,
+                            uint32_t* __restrict__ hcode, double2* __restrict__ spec, double* __restrict__ diag) {
+  const LatticeConst& g = *gp;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= bv.dim) return;
+  const RowState st = row_state(bv, g, row);
+  diag[row] = diag_energy(st.up, st.dn, g);
+  int64_t e = row_ptr[row], se = srow_ptr[row];
+  for (int spin = 0; spin < 2; ++spin) {
+    uint64_t occ = spin ? st.dn : st.up;
+    const uint64_t bits = occ;
+    while (occ) {
+      const int from = ctz(occ);
+      occ &= occ - 1;
+      for (int di = 0; di < g.ndir; ++di) {
+        const int dir = g.dirs[di];
+        const int to = g.nb[from][dir];
+        if ((bits >> to) & 1ull) continue;
+        int64_t col;
+        cd cf;
+        uint32_t code = 0;
+        bool special = false;
+        if (!hop_target_ex(bv, g, st.c, st.up, st.dn, st.dnf, spin, from, to, dir, &col, &cf, &code, &special))
+          continue;
+        hcol[e] = (int32_t)col;
+        if (special) {
+          spec[se] = make_double2(cf.re, cf.im);
+          hcode[e] = kSpecial | (uint32_t)se;
+          ++se;
+        } else {
+          hcode[e] = code;
+        }
+        ++e;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ cd hop_coef(uint32_t code, const double2* __restrict__ ptab,
+                                       const double2* __restrict__ spec) {
+  if (code & kSpecial) {
+    const double2 v = spec[code & (kSpecial - 1)];
+    return {v.x, v.y};
+  }
+  const double2 v = ptab[code & (kSpecial - 1)];
+  return (code & kNeg) ? cd{-v.x, -v.y} : cd{v.x, v.y};
+}
+
+// Batched gather over a hop list: one wavefront per row, lane l owns
+// columns cb + l + 64 v (v < VPL); the row's hop entries are wave-uniform
+// and read with scalar loads. With W < R the columns are tiled as in
+// k_apply_tiled (items = (group, column tile), XCD-contiguous).
+template <int VPL>
+__global__ void __launch_bounds__(256) k_apply_hl(int64_t dim, int R, int W, int J,
+                                                   const int64_t* __restrict__ offs, int n_canon,
+                                                   const int64_t* __restrict__ row_ptr,
+                                                   const int32_t* __restrict__ hcol,
+                                                   const uint32_t* __restrict__ hcode,
+                                                   const double2* __restrict__ ptab,
+                                                   const double2* __restrict__ spec,
+                                                   const double* __restrict__ diag,
+                                                   const double2* __restrict__ x, double2* __restrict__ y,
+                                                   double* __restrict__ partials, unsigned* counter,
+                                                   double* __restrict__ dot_out) {
+  __shared__ double wpart[4][64 * VPL];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  double part[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) part[v] = 0.0;
+  // work units: (row, column tile j); unit u -> j = u / dim within the XCD's
+  // range when tiled, so one XCD sweeps all rows of a group for a tile
+  const int64_t nunits = dim * J;
+  int64_t u0 = 0, u1 = nunits, ustep = (int64_t)gridDim.x * 4, ufirst = (int64_t)blockIdx.x * 4 + wave;
+  if (gridDim.x >= kXcd) {
+    const int xcd = blockIdx.x % kXcd;
+    const int l = blockIdx.x / kXcd;
+    const int nwg = (gridDim.x - xcd + kXcd - 1) / kXcd;
+    const int64_t chunk = (nunits + kXcd - 1) / kXcd;
+    u0 = xcd * chunk;
+    u1 = u0 + chunk < nunits ? u0 + chunk : nunits;
+    ufirst = u0 + (int64_t)l * 4 + wave;
+    ustep = (int64_t)nwg * 4;
+  }
+  for (int64_t u = ufirst; u < u1; u += ustep) {
+    int64_t row;
+    int j;
+    if (J == 1) {
+      row = u;
+      j = 0;
+    } else {
+      // unit order: group c, tile j, row within group
+      // find c with tile-major offsets: offs_t[c] = J * offs[c]
+      int lo = 0, hi = n_canon - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J * offs[mid] <= u) lo = mid;
+        else hi = mid - 1;
+      }
+      const int64_t base = J * offs[lo];
+      const int64_t gsz = offs[lo + 1] - offs[lo];
+      const int64_t k = u - base;
+      j = (int)(k / gsz);
+      row = offs[lo] + (k - (int64_t)j * gsz);
+    }
+    // the row is wave-uniform: keep it (and so the hop metadata loads) scalar
+    row = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)row)) |
+                    ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(row >> 32)) << 32));
+    const int cb = j * W;
+    const int wj = cb + W <= R ? W : R - cb;
+    const double dg = diag[row];
+    cd acc[VPL], xs[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      const int cc = lane + 64 * v;
+      xs[v] = {0.0, 0.0};
+      if (cc < wj) {
+        const double2 t = x[row * R + cb + cc];
+        xs[v] = {t.x, t.y};
+      }
+      acc[v] = {0.0 + dg * xs[v].re, 0.0 + dg * xs[v].im};
+    }
+    const int64_t e0 = row_ptr[row], e1 = row_ptr[row + 1];
+    int64_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+      int32_t cl[4];
+      cd cf[4];
+      double2 xv[4][VPL];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cl[q] = hcol[e + q];
+        cf[q] = hop_coef(hcode[e + q], ptab, spec);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const int cc = lane + 64 * v;
+          xv[q][v] = cc < wj ? x[(int64_t)cl[q] * R + cb + cc] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) acc[v] = cadd(acc[v], cmul(cf[q], {xv[q][v].x, xv[q][v].y}));
+    }
+    for (; e < e1; ++e) {
+      const int32_t cl = hcol[e];
+      const cd cf = hop_coef(hcode[e], ptab, spec);
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) {
+        const int cc = lane + 64 * v;
+        if (cc < wj) {
+          const double2 t = x[(int64_t)cl * R + cb + cc];
+          acc[v] = cadd(acc[v], cmul(cf, {t.x, t.y}));
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      const int cc = lane + 64 * v;
+      if (cc < wj) {
+        y[row * R + cb + cc] = make_double2(acc[v].re, acc[v].im);
+        // fused dot partials are column-resolved only when untiled; tiled
+        // runs reduce <x, y> in a separate pass (the caller passes no
+        // partials then)
+        if (partials) part[v] += xs[v].re * acc[v].re + xs[v].im * acc[v].im;
+      }
+    }
+  }
+  if (!partials) return;
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) wpart[wave][lane + 64 * v] = part[v];
+  __syncthreads();
+  for (int r = threadIdx.x; r < R; r += 256) {
+    double sum = 0.0;
+    for (int w = 0; w < 4; ++w) sum += wpart[w][r];
+    partials[(int64_t)blockIdx.x * R + r] = sum;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = threadIdx.x; r < R; r += 256) {
+    double sum = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) sum += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
+    dot_out[r] = sum;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Per-vector (R = 1) gather over a hop list: one row per thread.
+__global__ void __launch_bounds__(256) k_apply_hl_r1(int64_t dim, const int64_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ hcol,
+                                                      const uint32_t* __restrict__ hcode,
+                                                      const double2* __restrict__ ptab,
+                                                      const double2* __restrict__ spec,
+                                                      const double* __restrict__ diag,
+                                                      const double2* __restrict__ x, double2* __restrict__ y,
+                                                      double* __restrict__ partials, unsigned* counter,
+                                                      double* __restrict__ dot_out) {
+  double part = 0.0;
+  for (int64_t row = blockIdx.x * (int64_t)256 + threadIdx.x; row < dim; row += (int64_t)gridDim.x * 256) {
+    const double2 xt = x[row];
+    const double dg = diag[row];
+    cd acc = {0.0 + dg * xt.x, 0.0 + dg * xt.y};
+    const int64_t e1 = row_ptr[row + 1];
+    for (int64_t e = row_ptr[row]; e < e1; ++e) {
+      const double2 t = x[hcol[e]];
+      acc = cadd(acc, cmul(hop_coef(hcode[e], ptab, spec), {t.x, t.y}));
+    }
+    y[row] = make_double2(acc.re, acc.im);
+    part += xt.x * acc.re + xt.y * acc.im;
+  }
+  if (!partials) return;
+  __shared__ double red[256];
+  red[threadIdx.x] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < 256; ++i) s += red[i];
     partials[blockIdx.x] = s;
   }
   __threadfence();
@@ -445,9 +929,62 @@ ApplyPlan plan_apply(const LatticeConst& g, int nup, int ndn, int64_t dim, int R
   return p;
 }
 
+ApplyPlan plan_apply_tiled(const LatticeConst& g, int nup, int ndn, const int64_t* offs_host, int n_canon, int R,
+                           int W, int cus, std::vector<int64_t>& item_tiles) {
+  ApplyPlan p{};
+  p.R = R;
+  p.L = g.L;
+  p.W = W;
+  p.J = (R + W - 1) / W;
+  p.vpl = W <= 64 ? 1 : (W <= 128 ? 2 : 4);
+  int need = (W + p.vpl - 1) / p.vpl;
+  int lpr = 1;
+  while (lpr < need) lpr <<= 1;
+  if (lpr > 64) lpr = 64;
+  p.lpr = lpr;
+  p.nc = (nup + ndn) * g.ndir;
+  const int ncs = p.nc > 0 ? p.nc : 1;
+  int rpt = kThreads / lpr;
+  const size_t per_row = (size_t)ncs * (16 + 4) + 48 + 8 * (size_t)W;
+  while (rpt > 1 && per_row * rpt > 48 * 1024) rpt >>= 1;
+  p.rpt = rpt;
+  item_tiles.assign((size_t)n_canon * p.J + 1, 0);
+  int64_t t = 0;
+  for (int c = 0; c < n_canon; ++c) {
+    const int64_t rows = offs_host[c + 1] - offs_host[c];
+    const int64_t nt = (rows + rpt - 1) / rpt;
+    for (int j = 0; j < p.J; ++j) {
+      item_tiles[(size_t)c * p.J + j] = t;
+      t += nt;
+    }
+  }
+  item_tiles[(size_t)n_canon * p.J] = t;
+  p.ntiles = t;
+  p.n_items = n_canon * p.J;
+  int64_t grid = (int64_t)cus * 8;
+  if (grid > t) grid = t;
+  if (grid < 1) grid = 1;
+  p.grid = (int)grid;
+  const size_t slots = (size_t)rpt * ncs;
+  p.lds = table_lds_bytes(nup, ndn, g.L) + slots * 16 + ((slots + 1) & ~(size_t)1) * 4 + (size_t)rpt * (8 * 4) +
+          (size_t)rpt * W * 8 + (size_t)R * 8 + 64;
+  return p;
+}
+
 void launch_apply(const ApplyPlan& p, const LatticeConst* g, BlockView bv, const double* x, double* y,
                   double* partials, unsigned* counter, double* dot_out, hipStream_t s) {
   if (bv.dim == 0) return;
+  if (p.W > 0 && p.W < p.R) {
+    const double2* xx = reinterpret_cast<const double2*>(x);
+    double2* yy = reinterpret_cast<double2*>(y);
+    if (p.vpl == 1)
+      k_apply_tiled<1><<<p.grid, kThreads, p.lds, s>>>(g, bv, p.R, p.W, p.lpr, p.rpt, p.nc, p.J, p.item_tiles,
+                                                       p.n_items, p.ntiles, xx, yy, partials, counter, dot_out, p.L);
+    else
+      k_apply_tiled<2><<<p.grid, kThreads, p.lds, s>>>(g, bv, p.R, p.W, p.lpr, p.rpt, p.nc, p.J, p.item_tiles,
+                                                       p.n_items, p.ntiles, xx, yy, partials, counter, dot_out, p.L);
+    return;
+  }
   const int64_t ntiles = (bv.dim + p.rpt - 1) / p.rpt;
   const int bv_L = p.L;
   const double2* xx = reinterpret_cast<const double2*>(x);
@@ -493,6 +1030,49 @@ void launch_dense_build(const LatticeConst* g, const BlockView* bvs, int nblk, c
   dim3 grid(nblocks(max_dim, 64), nblk);
   k_dense_rows<<<grid, 64, 0, s>>>(g, bvs, moff, mats, max_dim);
   k_hermitise<<<grid, 64, 0, s>>>(bvs, moff, mats);
+}
+
+void launch_hops_count(const LatticeConst* g, BlockView bv, int64_t* cnt, int64_t* scnt, hipStream_t s) {
+  if (bv.dim == 0) return;
+  k_hops_count<<<nblocks(bv.dim, 128), 128, 0, s>>>(g, bv, cnt, scnt);
+}
+
+void launch_hops_fill(const LatticeConst* g, BlockView bv, const int64_t* row_ptr, const int64_t* srow_ptr,
+                      int32_t* hcol, uint32_t* hcode, double* spec, double* diag, hipStream_t s) {
+  if (bv.dim == 0) return;
+  k_hops_fill<<<nblocks(bv.dim, 128), 128, 0, s>>>(g, bv, row_ptr, srow_ptr, hcol, hcode,
+                                                   reinterpret_cast<double2*>(spec), diag);
+}
+
+void launch_apply_hl(const HopList& h, int64_t dim, int R, int W, const int64_t* offs, int n_canon, int cus,
+                     const double* x, double* y, double* partials, unsigned* counter, double* dot_out, hipStream_t s) {
+  if (dim == 0) return;
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  double2* yy = reinterpret_cast<double2*>(y);
+  const double2* pt = reinterpret_cast<const double2*>(h.ptab);
+  const double2* sp = reinterpret_cast<const double2*>(h.spec);
+  if (R == 1) {
+    int64_t grid = (dim + 255) / 256;
+    if (grid > (int64_t)cus * 8) grid = (int64_t)cus * 8;
+    k_apply_hl_r1<<<(unsigned)grid, 256, 0, s>>>(dim, h.row_ptr, h.col, h.code, pt, sp, h.diag, xx, yy, partials,
+                                                   counter, dot_out);
+    return;
+  }
+  if (W <= 0 || W >= R) W = R;
+  const int J = (R + W - 1) / W;
+  const int vpl = (W + 63) / 64;
+  int64_t grid = (dim * J + 3) / 4;
+  if (grid > (int64_t)cus * 8) grid = (int64_t)cus * 8;
+  if (grid < 1) grid = 1;
+  if (vpl <= 1)
+    k_apply_hl<1><<<(unsigned)grid, 256, 0, s>>>(dim, R, W, J, offs, n_canon, h.row_ptr, h.col, h.code, pt, sp,
+                                                 h.diag, xx, yy, partials, counter, dot_out);
+  else if (vpl == 2)
+    k_apply_hl<2><<<(unsigned)grid, 256, 0, s>>>(dim, R, W, J, offs, n_canon, h.row_ptr, h.col, h.code, pt, sp,
+                                                 h.diag, xx, yy, partials, counter, dot_out);
+  else
+    k_apply_hl<4><<<(unsigned)grid, 256, 0, s>>>(dim, R, W, J, offs, n_canon, h.row_ptr, h.col, h.code, pt, sp,
+                                                 h.diag, xx, yy, partials, counter, dot_out);
 }
 
 void launch_zero(double* p, int64_t n, hipStream_t s) {

@@ -146,6 +146,22 @@ struct ftlm_ctx {
   int last_reduce = FTLM_REDUCE_TREE;
   int prepared_R = 0;
   int prepared_layout = -1;  // 0: [dim][R], 1: [R][dim]
+  // H-apply configuration: path 0 automatic (hop list), 1 hop list, 2 on the
+  // fly; column tile width -1 automatic, 0 untiled, W > 0 tiles of W
+  int apply_path = 0;
+  int apply_w = -1;
+  // R = 1 (per-vector) runs: the on-the-fly kernel is the faster plain path
+  int apply_path_r1 = 2;
+  // hop list of the current block
+  DevBuf hl_rowptr, hl_col, hl_code, hl_spec, hl_diag, hl_cnt, hl_scnt, hl_srowptr;
+  int64_t hl_key = -1;
+  int64_t block_gen = 0;
+  HopList hl{};
+  DevBuf item_tiles;
+  int plan_key_w = -2, plan_key_R = 0;
+  int64_t plan_key_dim = -1;
+  int plan_key_mx = -1, plan_key_my = -1, plan_key_nup = -1, plan_key_ndn = -1;
+  ApplyPlan plan{};
   ftlm_stats stats{};
   // exact
   DevBuf ex_mats, ex_work, ex_eig, ex_moff, ex_dims, ex_eoff, ex_views;
@@ -250,6 +266,7 @@ void build_sector(ftlm_ctx* c, int nup, int ndn) {
     throw Error(FTLM_ERR_LIMIT, "spin configuration count exceeds 2^31");
   c->has_sector = false;
   c->cur.mx = c->cur.my = -1;
+  ++c->block_gen;
   c->nup = nup;
   c->ndn = ndn;
   c->n_upc = (int64_t)nu;
@@ -527,6 +544,124 @@ LanczosState lanczos_state(ftlm_ctx* c, int R, int steps) {
   return st;
 }
 
+// Column tile width for R vectors on the current block. Automatic choice:
+// the widest power of two (>= 8) whose largest group slab rows * W * 16 B
+// stays within 2 MB of the XCD's 4 MB L2; untiled when that covers R.
+int auto_tile(ftlm_ctx* c, int R, int64_t max_group) {
+  if (R < 16) return 0;
+  int w = 8;
+  while (w * 2 < R && (double)max_group * (w * 2) * 16.0 <= 2.0 * 1024 * 1024) w *= 2;
+  return w >= R ? 0 : w;
+}
+
+const ApplyPlan& apply_plan(ftlm_ctx* c, int R, int w_req) {
+  BlockTables& b = c->cur;
+  std::vector<int64_t> offs(c->n_canon + 1);
+  int w = w_req;
+  if (w < 0 || (w > 0 && w >= R) || c->plan_key_dim != b.dim || c->plan_key_mx != b.mx || c->plan_key_my != b.my ||
+      c->plan_key_nup != c->nup || c->plan_key_ndn != c->ndn || c->plan_key_R != R || c->plan_key_w != w_req) {
+    HIPCHECK(hipMemcpy(offs.data(), b.offs.p, 8 * (c->n_canon + 1), hipMemcpyDeviceToHost));
+  } else {
+    return c->plan;
+  }
+  int64_t maxg = 0;
+  for (int k = 0; k < c->n_canon; ++k) maxg = std::max(maxg, offs[k + 1] - offs[k]);
+  if (w < 0) w = auto_tile(c, R, maxg);
+  if (w <= 0 || w >= R) {
+    c->plan = plan_apply(c->g, c->nup, c->ndn, b.dim, R, c->cus);
+    c->plan.W = 0;
+  } else {
+    std::vector<int64_t> it;
+    c->plan = plan_apply_tiled(c->g, c->nup, c->ndn, offs.data(), c->n_canon, R, w, c->cus, it);
+    int64_t* d = c->item_tiles.as<int64_t>(it.size());
+    HIPCHECK(hipMemcpy(d, it.data(), it.size() * 8, hipMemcpyHostToDevice));
+    c->plan.item_tiles = d;
+  }
+  c->plan_key_dim = b.dim;
+  c->plan_key_mx = b.mx;
+  c->plan_key_my = b.my;
+  c->plan_key_nup = c->nup;
+  c->plan_key_ndn = c->ndn;
+  c->plan_key_R = R;
+  c->plan_key_w = w_req;
+  return c->plan;
+}
+
+// Hop list of the current block (built on the device once per block).
+const HopList& ensure_hops(ftlm_ctx* c) {
+  if (c->hl_key == c->block_gen) return c->hl;
+  BlockTables& b = c->cur;
+  const int64_t n = b.dim;
+  hipStream_t s = c->stream;
+  int64_t* cnt = c->hl_cnt.as<int64_t>(n + 1);
+  int64_t* scnt = c->hl_scnt.as<int64_t>(n + 1);
+  int64_t* rp = c->hl_rowptr.as<int64_t>(n + 1);
+  int64_t* srp = c->hl_srowptr.as<int64_t>(n + 1);
+  launch_hops_count(gdev(c), b.view, cnt, scnt, s);
+  launch_prefix64(cnt, n, rp, s);
+  launch_prefix64(scnt, n, srp, s);
+  HIPCHECK(hipGetLastError());
+  const int64_t nnz = read1<int64_t>(c, rp + n);
+  const int64_t nsp = read1<int64_t>(c, srp + n);
+  if (nnz > INT64_C(0x7fffffff) * 8) throw Error(FTLM_ERR_LIMIT, "hop list too large");
+  int32_t* col = c->hl_col.as<int32_t>(nnz + 1);
+  uint32_t* code = c->hl_code.as<uint32_t>(nnz + 1);
+  double* spec = c->hl_spec.as<double>(2 * nsp + 2);
+  double* diag = c->hl_diag.as<double>(n + 1);
+  launch_hops_fill(gdev(c), b.view, rp, srp, col, code, spec, diag, s);
+  HIPCHECK(hipGetLastError());
+  sync(c);
+  c->hl.row_ptr = rp;
+  c->hl.col = col;
+  c->hl.code = code;
+  c->hl.spec = spec;
+  c->hl.diag = diag;
+  c->hl.ptab = b.view.ptab;
+  c->hl_key = c->block_gen;
+  return c->hl;
+}
+
+struct ApplyCfg {
+  int path;  // 1 hop list, 2 on the fly
+  int w;     // 0 untiled, else tile width
+};
+
+ApplyCfg resolve_cfg(ftlm_ctx* c, int R, int path, int w) {
+  ApplyCfg a;
+  if (R == 1 && path == c->apply_path) path = c->apply_path_r1;
+  a.path = path == 2 ? 2 : 1;
+  if (w < 0) {
+    std::vector<int64_t> offs(c->n_canon + 1);
+    HIPCHECK(hipMemcpy(offs.data(), c->cur.offs.p, 8 * (c->n_canon + 1), hipMemcpyDeviceToHost));
+    int64_t maxg = 0;
+    for (int k = 0; k < c->n_canon; ++k) maxg = std::max(maxg, offs[k + 1] - offs[k]);
+    w = a.path == 1 ? 0 : auto_tile(c, R, maxg);
+  }
+  a.w = (w >= R) ? 0 : w;
+  return a;
+}
+
+// y = H x for R columns with configuration cfg; partials/dot_out may be null.
+// Returns true when <x, y> was reduced into dot_out by the apply itself.
+bool do_apply(ftlm_ctx* c, const ApplyCfg& cfg, int R, const double* x, double* y, double* partials,
+              unsigned* counter, double* dot_out) {
+  const int64_t dim = c->cur.dim;
+  if (cfg.path == 1) {
+    const HopList& h = ensure_hops(c);
+    const bool fused = cfg.w == 0 || R == 1;
+    launch_apply_hl(h, dim, R, cfg.w, c->cur.offs.ptr<int64_t>(), c->n_canon, c->cus, x, y,
+                    fused ? partials : nullptr, counter, dot_out, c->stream);
+    return fused && partials;
+  }
+  if (R == 1) {
+    launch_apply_r1(gdev(c), c->g.L, c->cur.view, x, y, c->cus, partials, counter, dot_out, c->stream);
+    return partials != nullptr;
+  }
+  const ApplyPlan& p = apply_plan(c, R, cfg.w);
+  launch_apply(p, gdev(c), c->cur.view, x, y, partials, counter, dot_out, c->stream);
+  return partials != nullptr;
+}
+
 // distinct non-zero-amplitude neighbour sites of one site (translation
 // invariant), for the algorithmic byte count
 double hbar(ftlm_ctx* c) {
@@ -550,11 +685,10 @@ void lanczos_run(ftlm_ctx* c, int R, int steps, int reduce, double* x0, double* 
   const int64_t dim = c->cur.dim;
   const int kdim = (int)std::min<int64_t>(steps, dim);
   hipStream_t s = c->stream;
-  BlockView bv = c->cur.view;
-  ApplyPlan plan = plan_apply(c->g, c->nup, c->ndn, dim, R, c->cus);
   if (r1_kernel && R != 1) throw Error(FTLM_ERR_ARG, "one-row kernel needs R = 1");
-  const int gridp = r1_kernel ? (int)std::min<int64_t>((dim + 255) / 256, (int64_t)c->cus * 8) : plan.grid;
-  double* partials = c->part.as<double>((size_t)std::max(gridp, 8192) * R + 64);
+  const ApplyCfg cfg = resolve_cfg(c, R, c->apply_path, c->apply_w);
+  if (cfg.path == 1) ensure_hops(c);
+  double* partials = c->part.as<double>((size_t)std::max(c->cus * 8, 8192) * R + 64);
   unsigned* counter = c->counter.as<unsigned>(4);
   HIPCHECK(hipMemsetAsync(counter, 0, 16, s));
   HIPCHECK(hipMemsetAsync(x1, 0, (size_t)dim * R * 16, s));
@@ -569,12 +703,10 @@ void lanczos_run(ftlm_ctx* c, int R, int steps, int reduce, double* x0, double* 
   const bool tree = reduce == FTLM_REDUCE_TREE;
   for (int k = 0; k < kdim; ++k) {
     if (timed) HIPCHECK(hipEventRecord(ev[2 * k], s));
-    if (r1_kernel)
-      launch_apply_r1(gdev(c), c->g.L, bv, x0, w, c->cus, tree ? partials : nullptr, counter, st.dot, s);
-    else
-      launch_apply(plan, gdev(c), bv, x0, w, tree ? partials : nullptr, counter, st.dot, s);
+    const bool fused = do_apply(c, cfg, R, x0, w, tree ? partials : nullptr, counter, st.dot);
     if (timed) HIPCHECK(hipEventRecord(ev[2 * k + 1], s));
     if (tree) {
+      if (!fused) launch_dot_tree(dim, R, x0, w, partials, counter, st.dot, rgrid, s);
       launch_recur_tree(dim, R, w, x0, x1, k, kdim, st, partials, counter, rgrid, s);
     } else {
       launch_dot_serial(dim, R, x0, w, st.dot, s);
@@ -753,6 +885,7 @@ int ftlm_block(ftlm_ctx* ctx, int mx, int my, int64_t* dim) {
     need_sector(ctx);
     HIPCHECK(hipSetDevice(ctx->device));
     ctx->prepared_R = 0;
+    ++ctx->block_gen;
     build_block(ctx, ctx->cur, mx, my);
     if (dim) *dim = ctx->cur.dim;
   });
@@ -823,6 +956,7 @@ int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y) {
     need_block(ctx);
     check_R(R);
     if (kernel == 1 && R != 1) throw Error(FTLM_ERR_ARG, "kernel 1 handles R = 1 only");
+    if (kernel != 0 && kernel != 1) throw Error(FTLM_ERR_ARG, "kernel must be 0 (configured) or 1 (one row per thread)");
     const int64_t n = ctx->cur.dim;
     if (n == 0) return;
     ensure_vectors(ctx, R);
@@ -832,8 +966,7 @@ int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y) {
     if (kernel == 1) {
       launch_apply_r1(gdev(ctx), ctx->g.L, ctx->cur.view, dx, dy, ctx->cus, nullptr, nullptr, nullptr, ctx->stream);
     } else {
-      const ApplyPlan p = plan_apply(ctx->g, ctx->nup, ctx->ndn, n, R, ctx->cus);
-      launch_apply(p, gdev(ctx), ctx->cur.view, dx, dy, nullptr, nullptr, nullptr, ctx->stream);
+      do_apply(ctx, resolve_cfg(ctx, R, ctx->apply_path, ctx->apply_w), R, dx, dy, nullptr, nullptr, nullptr);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(y, dy, (size_t)n * R * 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -1039,6 +1172,87 @@ int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t*
     for (int i = 0; i <= nblk; ++i) offsets[i] = eoff[i];
     for (int64_t i = 0; i < eoff[nblk]; ++i)
       if (std::isnan(eig_out[i])) throw Error(FTLM_ERR_DEVICE, "exact eigensolver did not converge");
+  });
+}
+
+int ftlm_set_apply_config(ftlm_ctx* ctx, int path, int W) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    if (path < 0 || path > 2) throw Error(FTLM_ERR_ARG, "path must be 0 (automatic), 1 (hop list) or 2 (on the fly)");
+    if (W < -1) throw Error(FTLM_ERR_ARG, "tile width must be -1 (automatic), 0 (untiled) or positive");
+    ctx->apply_path = path;
+    ctx->apply_w = W;
+    ctx->apply_path_r1 = path == 0 ? 2 : path;
+    ctx->plan_key_dim = -1;
+  });
+}
+
+int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W) {
+  return guarded([&] {
+    if (!ctx || !path || !W) throw Error(FTLM_ERR_ARG, "null argument");
+    need_block(ctx);
+    const ApplyCfg a = resolve_cfg(ctx, R, ctx->apply_path, ctx->apply_w);
+    *path = a.path;
+    *W = a.w;
+  });
+}
+
+int ftlm_autotune(ftlm_ctx* ctx, int R, int* best_path, int* best_w, double* best_ms, int* ncand, int* cand_path,
+                  int* cand_w, double* cand_ms) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_block(ctx);
+    check_R(R);
+    const int64_t n = ctx->cur.dim;
+    if (n == 0) throw Error(FTLM_ERR_ARG, "empty block");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ensure_vectors(ctx, R);
+    std::vector<ApplyCfg> cands = {{1, 0}};
+    for (int w : {64, 32, 16, 8})
+      if (w < R) cands.push_back({1, w});
+    cands.push_back({2, 0});
+    if (R == 1) cands = {{1, 0}, {2, 0}};
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    double bms = 1e300;
+    ApplyCfg best = cands[0];
+    int k = 0;
+    for (const ApplyCfg& cf : cands) {
+      const double* x = ctx->vec[0].ptr<double>();
+      double* y = ctx->vec[1].ptr<double>();
+      do_apply(ctx, cf, R, x, y, nullptr, nullptr, nullptr);  // warm (and build the hop list)
+      HIPCHECK(hipEventRecord(e0, ctx->stream));
+      for (int rep = 0; rep < 2; ++rep) do_apply(ctx, cf, R, x, y, nullptr, nullptr, nullptr);
+      HIPCHECK(hipEventRecord(e1, ctx->stream));
+      HIPCHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / 2.0;
+      if (cand_w && cand_ms && cand_path && ncand && k < *ncand) {
+        cand_path[k] = cf.path;
+        cand_w[k] = cf.w;
+        cand_ms[k] = per;
+      }
+      ++k;
+      if (per < bms) {
+        bms = per;
+        best = cf;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ncand) *ncand = k;
+    if (R == 1) {
+      ctx->apply_path_r1 = best.path;
+    } else {
+      ctx->apply_path = best.path;
+      ctx->apply_w = best.w;
+    }
+    ctx->plan_key_dim = -1;
+    if (best_path) *best_path = best.path;
+    if (best_w) *best_w = best.w;
+    if (best_ms) *best_ms = bms;
   });
 }
 

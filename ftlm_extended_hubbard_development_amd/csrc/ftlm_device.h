@@ -122,6 +122,25 @@ struct RowState {
   double dnf;
 };
 
+// Row state when its canonical up group c is already known.
+__device__ __forceinline__ RowState row_state_in(const BlockView& bv, const LatticeConst& g, int c, int64_t row) {
+  RowState s;
+  s.c = c;
+  s.up = bv.can_bits[c];
+  const int64_t local = row - bv.offs[c];
+  const int x = bv.can_nt[c];
+  int64_t drank;
+  if (x < 0) {
+    drank = local;
+    s.dnf = 1.0;
+  } else {
+    drank = bv.nt_list[bv.nt_listoff[x] + local];
+    s.dnf = bv.nt_dnf[(int64_t)x * bv.n_dnc + drank];
+  }
+  s.dn = colex_unrank(drank, bv.ndn, g.L, bv.binom);
+  return s;
+}
+
 __device__ __forceinline__ RowState row_state(const BlockView& bv, const LatticeConst& g, int64_t row) {
   // last c with offs[c] <= row
   int lo = 0, hi = bv.n_canon - 1;
@@ -156,9 +175,9 @@ __device__ __forceinline__ RowState row_state(const BlockView& bv, const Lattice
 // ptab[dir][shift] holds amp(dir) (*) phase[shift] computed on the host by the
 // same std::complex expression, signs are exact negations and a norm ratio
 // of exactly 1 is skipped as the exact no-op it is.
-__device__ __forceinline__ bool hop_target(const BlockView& bv, const LatticeConst& g, int c_row, uint64_t up,
-                                           uint64_t dn, double dnf_row, int spin, int from, int to, int dir,
-                                           int64_t* col, cd* coef) {
+__device__ __forceinline__ bool hop_target_ex(const BlockView& bv, const LatticeConst& g, int c_row, uint64_t up,
+                                              uint64_t dn, double dnf_row, int spin, int from, int to, int dir,
+                                              int64_t* col, cd* coef, uint32_t* code, bool* special) {
   const uint64_t bits = spin ? dn : up;
   int sgn = hop_fsign(bits, from, to);
   const uint64_t moved = bits ^ (1ull << from) ^ (1ull << to);
@@ -235,13 +254,22 @@ __device__ __forceinline__ bool hop_target(const BlockView& bv, const LatticeCon
   }
   const double* pt = bv.ptab + 2 * (dir * g.L + jsel);
   cd v = {pt[0], pt[1]};
-  if (dnf_t != 1.0 || dnf_row != 1.0) {
+  const bool ratio_needed = dnf_t != 1.0 || dnf_row != 1.0;
+  if (ratio_needed) {
     const double ratio = dnf_t / dnf_row;
     v = {v.re * ratio, v.im * ratio};
   }
   if (sgn < 0) v = {-v.re, -v.im};
   *coef = v;
+  if (code) *code = (uint32_t)(dir * g.L + jsel) | (sgn < 0 ? (1u << 31) : 0u);
+  if (special) *special = ratio_needed;
   return true;
+}
+
+__device__ __forceinline__ bool hop_target(const BlockView& bv, const LatticeConst& g, int c_row, uint64_t up,
+                                           uint64_t dn, double dnf_row, int spin, int from, int to, int dir,
+                                           int64_t* col, cd* coef) {
+  return hop_target_ex(bv, g, c_row, up, dn, dnf_row, spin, from, to, dir, col, coef, nullptr, nullptr);
 }
 
 }  // namespace ftlm

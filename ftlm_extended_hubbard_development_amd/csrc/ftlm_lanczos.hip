@@ -149,15 +149,31 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
       c3_[j] = st.bprev[r] * st.sprev[r];
     }
   }
-  for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += (int64_t)gridDim.x * rpi) {
+  // four rows in flight per thread; each row's elements are processed in the
+  // same order as a one-row loop would, so the sums do not change
+  const int64_t stride = (int64_t)gridDim.x * rpi;
+  for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += 4 * stride) {
     for (int j = 0; j < 4; ++j) {
       if (!run[j]) continue;
-      const int64_t e = row * R + r0 + 256 * j;
-      const double2 vy = y[e], vx = x[e], vp = xp[e];
-      const double re = (s_[j] * vy.x - c2_[j] * vx.x) - c3_[j] * vp.x;
-      const double im = (s_[j] * vy.y - c2_[j] * vx.y) - c3_[j] * vp.y;
-      xp[e] = make_double2(re, im);
-      acc[j] += re * re + im * im;
+      double2 vy[4], vx[4], vp[4];
+      int64_t e[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        e[u] = (row + u * stride) * R + r0 + 256 * j;
+        if (row + u * stride < dim) {
+          vy[u] = y[e[u]];
+          vx[u] = x[e[u]];
+          vp[u] = xp[e[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (row + u * stride >= dim) continue;
+        const double re = (s_[j] * vy[u].x - c2_[j] * vx[u].x) - c3_[j] * vp[u].x;
+        const double im = (s_[j] * vy[u].y - c2_[j] * vx[u].y) - c3_[j] * vp[u].y;
+        xp[e[u]] = make_double2(re, im);
+        acc[j] += re * re + im * im;
+      }
     }
   }
   for (int j = 0; j < 4; ++j) sh[j][tid] = acc[j];
@@ -195,6 +211,63 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
     st.sprev[r] = sc;
     st.scale[r] = 1.0 / b;
     st.bprev[r] = b;
+  }
+  if (tid == 0) *counter = 0u;
+}
+
+// Per-column Re<x, y> with the column ownership of k_recur_tree.
+__global__ void __launch_bounds__(256) k_dot_tree(int64_t dim, int R, const double2* __restrict__ x,
+                                                  const double2* __restrict__ y, double* partials, unsigned* counter,
+                                                  double* out) {
+  __shared__ double sh[4][256];
+  const int tid = threadIdx.x;
+  const int rpi = R <= 256 ? 256 / R : 1;
+  const int cpt = R <= 256 ? 1 : (R + 255) / 256;
+  const int rsub = R <= 256 ? tid / R : 0;
+  const int r0 = R <= 256 ? tid % R : tid;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  bool run[4];
+  for (int j = 0; j < 4; ++j) run[j] = rsub < rpi && j < cpt && r0 + 256 * j < R;
+  const int64_t stride = (int64_t)gridDim.x * rpi;
+  for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += 4 * stride) {
+    for (int j = 0; j < 4; ++j) {
+      if (!run[j]) continue;
+      double2 a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (row + u * stride < dim) {
+          const int64_t e = (row + u * stride) * R + r0 + 256 * j;
+          a[u] = x[e];
+          b[u] = y[e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (row + u * stride < dim) acc[j] += a[u].x * b[u].x + a[u].y * b[u].y;
+    }
+  }
+  for (int j = 0; j < 4; ++j) sh[j][tid] = acc[j];
+  __syncthreads();
+  for (int r = tid; r < R; r += 256) {
+    double s = 0.0;
+    if (R <= 256) {
+      for (int q = r; q < rpi * R; q += R) s += sh[0][q];
+    } else {
+      s = sh[r / 256][r % 256];
+    }
+    partials[(int64_t)blockIdx.x * R + r] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  __shared__ unsigned last;
+  if (tid == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = tid; r < R; r += 256) {
+    double s = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
+    out[r] = s;
   }
   if (tid == 0) *counter = 0u;
 }
@@ -615,6 +688,12 @@ void launch_recur_tree(int64_t dim, int R, const double* y, const double* x, dou
   k_recur_tree<<<grid, 256, 0, s>>>(
       dim, R, reinterpret_cast<const double2*>(y), reinterpret_cast<const double2*>(x),
       reinterpret_cast<double2*>(xp), k, kdim, st, partials, counter);
+}
+
+void launch_dot_tree(int64_t dim, int R, const double* x, const double* y, double* partials, unsigned* counter,
+                     double* out, int grid, hipStream_t s) {
+  k_dot_tree<<<grid, 256, 0, s>>>(dim, R, reinterpret_cast<const double2*>(x), reinterpret_cast<const double2*>(y),
+                                   partials, counter, out);
 }
 
 void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src, double* x, hipStream_t s) {

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "ftlm_types.h"
 
 namespace ftlm {
@@ -46,13 +48,35 @@ struct ApplyPlan {
   int grid;     // workgroups (persistent)
   int L;        // sites
   size_t lds;   // dynamic LDS bytes
+  // column tiling (W < R): tile width, tiles per row, work items
+  int W = 0, J = 1, n_items = 0;
+  int64_t ntiles = 0;
+  const int64_t* item_tiles = nullptr;  // device: first tile of each item
 };
 ApplyPlan plan_apply(const LatticeConst& g, int nup, int ndn, int64_t dim, int R, int cus);
+// Column-tiled plan: item_tiles (host) receives the first tile of every
+// (group, column tile) item; the caller uploads it and sets p.item_tiles.
+ApplyPlan plan_apply_tiled(const LatticeConst& g, int nup, int ndn, const int64_t* offs_host, int n_canon, int R,
+                           int W, int cus, std::vector<int64_t>& item_tiles);
 void launch_apply(const ApplyPlan& p, const LatticeConst* g, BlockView bv, const double* x, double* y,
                   double* partials, unsigned* counter, double* dot_out, hipStream_t s);
 void launch_apply_r1(const LatticeConst* g, int L, BlockView bv, const double* x, double* y, int cus,
                      double* partials, unsigned* counter, double* dot_out, hipStream_t s);
 void launch_zero(double* p, int64_t n, hipStream_t s);
+// Hop lists (built once per block, reused by every apply).
+struct HopList {
+  const int64_t* row_ptr = nullptr;  // dim + 1
+  const int32_t* col = nullptr;
+  const uint32_t* code = nullptr;
+  const double* spec = nullptr;      // complex coefficients of special hops
+  const double* diag = nullptr;      // diagonal energies
+  const double* ptab = nullptr;      // [8][L] complex amp * phase
+};
+void launch_hops_count(const LatticeConst* g, BlockView bv, int64_t* cnt, int64_t* scnt, hipStream_t s);
+void launch_hops_fill(const LatticeConst* g, BlockView bv, const int64_t* row_ptr, const int64_t* srow_ptr,
+                      int32_t* hcol, uint32_t* hcode, double* spec, double* diag, hipStream_t s);
+void launch_apply_hl(const HopList& h, int64_t dim, int R, int W, const int64_t* offs, int n_canon, int cus,
+                     const double* x, double* y, double* partials, unsigned* counter, double* dot_out, hipStream_t s);
 // CSR export: counts per row (pass 1) and entries (pass 2).
 void launch_csr_count(const LatticeConst* g, BlockView bv, int all_dirs, int32_t* counts, hipStream_t s);
 void launch_csr_fill(const LatticeConst* g, BlockView bv, int all_dirs, const int64_t* row_ptr, int32_t* col,
@@ -83,6 +107,10 @@ void launch_normalize(int64_t dim, int R, double* q, const double* beta_k, const
                       const int32_t* mcount, hipStream_t s);
 void launch_recur_tree(int64_t dim, int R, const double* y, const double* x, double* xp, int k, int kdim,
                        LanczosState st, double* partials, unsigned* counter, int grid, hipStream_t s);
+// per-column Re<x, y>, tree order (per workgroup, then workgroups in index
+// order in the last workgroup)
+void launch_dot_tree(int64_t dim, int R, const double* x, const double* y, double* partials, unsigned* counter,
+                     double* out, int grid, hipStream_t s);
 void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src, double* x, hipStream_t s);
 void launch_ritz_ql(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
                     double* eig, double* w, double* scratch, int32_t* iscratch, int32_t* fail, hipStream_t s);

@@ -87,7 +87,8 @@ int ftlm_block_csr(ftlm_ctx* ctx, int all_dirs, int64_t* nnz, int32_t* row_ptr, 
                    double* diag);
 
 /* y = H x for R vectors of the current block (host buffers, 2*dim*R each).
- * kernel: 0 = batched block-vector kernel, 1 = one-row-per-thread kernel. */
+ * kernel: 0 = the context's apply configuration, 1 = the on-the-fly
+ * one-row-per-thread kernel (R = 1). */
 int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y);
 
 /* Seeded unit-norm complex Gaussian start vector of one sample
@@ -121,6 +122,23 @@ int ftlm_prepare(ftlm_ctx* ctx, int R, uint64_t block_seed);
 int ftlm_run(ftlm_ctx* ctx, int R, int steps, int reduce, int per_vector_column);
 int ftlm_finish(ftlm_ctx* ctx, int R, int steps, double* eig, double* w, int64_t* count);
 int ftlm_last_stats(ftlm_ctx* ctx, ftlm_stats* out);
+
+/* H-apply configuration (no reference counterpart; engine tuning knobs).
+ * path: 0 automatic (= 1), 1 hop list -- the block's hops (target row and
+ * coefficient code, 8 bytes each) are built once on the device and every
+ * apply is a pure gather -- or 2 on the fly -- hops are recomputed from the
+ * basis tables inside each apply. W: -1 automatic, 0 untiled, W > 0 column
+ * tiles of W vectors. ftlm_get_apply_config reports what R vectors on the
+ * current block use. */
+int ftlm_set_apply_config(ftlm_ctx* ctx, int path, int W);
+int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W);
+
+/* Autotuner: times the candidate configurations (hop list untiled and with
+ * tiles 64/32/16/8 below R, on the fly untiled) on the current block with R
+ * vectors and keeps the fastest. On input *ncand is the capacity of the
+ * cand_* arrays (which may be NULL); on output the number timed. */
+int ftlm_autotune(ftlm_ctx* ctx, int R, int* best_path, int* best_w, double* best_ms, int* ncand, int* cand_path,
+                  int* cand_w, double* cand_ms);
 
 /* diagonalize_block_exact for blocks (mx[i], my[i]) of the current sector,
  * batched in one launch: eig_out receives the ascending eigenvalues of each

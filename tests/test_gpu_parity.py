@@ -53,12 +53,19 @@ def engine(pkg, name):
     return _engines[name]
 
 
+_oracle_blocks = {}
+
+
 def both(pkg, orc, name, nup, ndn, mx, my):
     say(f"case {name} ({nup},{ndn},{mx},{my})")
     eng = engine(pkg, name)
-    eng.sector(nup, ndn)
+    if (eng.nup, eng.ndn) != (nup, ndn):
+        eng.sector(nup, ndn)
     dim = eng.block(mx, my)
-    blk = orc.block(oracle_lib.model(**MODELS[name]), nup, ndn, mx, my)
+    key = (name, nup, ndn, mx, my)
+    if key not in _oracle_blocks:
+        _oracle_blocks[key] = orc.block(oracle_lib.model(**MODELS[name]), nup, ndn, mx, my)
+    blk = _oracle_blocks[key]
     assert dim == blk.dim
     return eng, blk
 
@@ -100,6 +107,59 @@ def test_apply_bitexact(pkg, orc, case, R):
     assert np.array_equal(y, yo)
     if R == 1:
         assert np.array_equal(eng.apply(x, kernel=1), yo)
+
+
+@pytest.mark.parametrize("path,W", [(1, 0), (1, 8), (1, 16), (1, 64), (2, 0), (2, 8), (2, 16), (2, 32)])
+@pytest.mark.parametrize("R", [1, 5, 33, 64, 128, 130])
+@pytest.mark.parametrize("case", [("4x2", 4, 4, 0, 0), ("3x3tw", 4, 3, 1, 2), ("4x4", 3, 2, 1, 3),
+                                  ("4x4", 4, 4, 2, 0)], ids=lambda c: "-".join(map(str, c)))
+def test_apply_configs_bitexact(pkg, orc, case, R, path, W):
+    """Hop-list and on-the-fly kernels, untiled and column-tiled: the same
+    bits as the reference's apply for every configuration."""
+    eng, blk = both(pkg, orc, *case)
+    rng = np.random.default_rng(R + W)
+    x = rng.standard_normal((blk.dim, R)) + 1j * rng.standard_normal((blk.dim, R))
+    eng.set_apply_config(path, W)
+    try:
+        assert np.array_equal(eng.apply(x), blk.apply(x))
+    finally:
+        eng.set_apply_config(0, -1)
+
+
+def test_autotune_and_configured_lanczos(pkg, orc):
+    """The autotuner picks a candidate; tree-order Lanczos through every
+    apply configuration matches the oracle before ghost onset."""
+    case = ("4x4", 3, 2, 1, 3)
+    eng, blk = both(pkg, orc, *case)
+    bs = pkg.block_seed(12345, *case[1:])
+    R = 32
+    best, ms, cands = eng.autotune(R)
+    assert ms > 0 and len(cands) >= 3
+    try:
+        for path, W in ((1, 0), (1, 8), (2, 0), (2, 8)):
+            eng.set_apply_config(path, W)
+            a, b, m = eng.lanczos(R, 40, bseed=bs, reduce=pkg.REDUCE_TREE)
+            for r in range(0, R, 7):
+                oa, ob = orc.lanczos(blk, 40, bs, r)
+                assert np.allclose(a[:30, r], oa[:30], rtol=1e-12, atol=1e-12)
+                assert np.allclose(b[:30, r], ob[:30], rtol=1e-12, atol=0)
+    finally:
+        eng.set_apply_config(0, -1)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+def test_serial_bitexact_both_paths(pkg, orc, path):
+    case = ("4x4", 3, 2, 1, 3)
+    eng, blk = both(pkg, orc, *case)
+    bs = pkg.block_seed(12345, *case[1:])
+    eng.set_apply_config(path, -1)
+    try:
+        for pv in (False, True):
+            e, w = eng.ftlm_block(4, 60, bs, reduce=pkg.REDUCE_SERIAL, per_vector=pv)
+            oe, ow = orc.ftlm_block(blk, 4, 60, bs)
+            assert np.array_equal(e, oe) and np.array_equal(w, ow)
+    finally:
+        eng.set_apply_config(0, -1)
 
 
 @pytest.mark.parametrize("case", [("4x2", 4, 4, 0, 0), ("3x3tw", 4, 3, 1, 2), ("4x4", 3, 2, 1, 3),
