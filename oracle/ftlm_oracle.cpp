@@ -923,6 +923,14 @@ void orc_lattice(int lx, int ly, int* nb, int* bonds, int* nbonds) {
 }
 
 unsigned long long orc_mix_u64(unsigned long long z) { return orc::mix(z); }
+// n-th output (1-based) of std::mt19937_64 seeded with `seed` (known-answer
+// pin of the engine behind the start vectors).
+unsigned long long orc_mt_nth(unsigned long long seed, long long n) {
+  std::mt19937_64 rng(seed);
+  unsigned long long v = 0;
+  for (long long i = 0; i < n; ++i) v = rng();
+  return v;
+}
 unsigned long long orc_block_seed(unsigned long long seed, int nup, int ndn, int mx, int my) {
   return orc::block_seed(seed, nup, ndn, mx, my);
 }
