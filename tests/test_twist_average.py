@@ -1,0 +1,78 @@
+"""Twist averaging (restatement of scripts/average_twist_outputs.py) pinned
+by hand-computed cases: for two twists with known Z_t, <N>_t, <N^2>_t the
+partition-sum result must equal the statistics of the merged ensemble, and
+observable-average the plain means."""
+import csv
+import math
+
+import pytest
+
+from ftlm_extended_hubbard_development_amd import twist_average as ta
+
+
+def write_csv(path, rows, with_beta=True):
+    fields = (["beta"] if with_beta else []) + ["mu", "n", "charge_correlation", "compressibility",
+                                                "partition_like", "log_partition"]
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        for r in rows:
+            w.writerow({k: r[k] for k in fields})
+
+
+def ensemble(L, beta, mu, levels):
+    """levels: list of (E, N, multiplicity) -> CSV row of that twist."""
+    z = sum(g * math.exp(-beta * (e - mu * n)) for e, n, g in levels)
+    n1 = sum(g * n * math.exp(-beta * (e - mu * n)) for e, n, g in levels) / z
+    n2 = sum(g * n * n * math.exp(-beta * (e - mu * n)) for e, n, g in levels) / z
+    cc = (n2 - n1 * n1) / L
+    return {"beta": beta, "mu": mu, "n": n1 / L, "charge_correlation": cc, "compressibility": beta * cc,
+            "partition_like": z, "log_partition": math.log(z)}, z, n1, n2
+
+
+def test_partition_sum_equals_merged_ensemble(tmp_path):
+    L, beta = 4, 2.5
+    twists = [[(0.0, 0, 1), (-1.0, 1, 4), (0.5, 2, 6)], [(0.0, 0, 1), (-0.7, 1, 4), (-0.2, 2, 3)]]
+    files = []
+    for t, lev in enumerate(twists):
+        rows = [ensemble(L, beta, mu, lev)[0] for mu in (-1.0, 0.0, 0.75)]
+        p = tmp_path / f"t{t}.csv"
+        write_csv(p, rows)
+        files.append(str(p))
+    out = ta.combine(ta.read_rows(files), "partition-sum", L)
+    for rec, mu in zip(out, (-1.0, 0.0, 0.75)):
+        merged = twists[0] + twists[1]
+        _, z, n1, n2 = ensemble(L, beta, mu, merged)
+        assert rec["twist_count"] == 2 and rec["mode"] == "partition-sum"
+        assert abs(rec["n"] - n1 / L) < 1e-13
+        assert abs(rec["charge_correlation"] - (n2 - n1 * n1) / L) < 1e-13
+        assert abs(rec["compressibility"] - beta * (n2 - n1 * n1) / L) < 1e-12
+        assert abs(rec["log_partition"] - math.log(z)) < 1e-13
+
+
+def test_observable_average(tmp_path):
+    a = [{"mu": 0.0, "n": 0.25, "charge_correlation": 0.1, "compressibility": 0.2, "partition_like": 1,
+          "log_partition": 1.0}]
+    b = [{"mu": 0.0, "n": 0.75, "charge_correlation": 0.3, "compressibility": 0.6, "partition_like": 1,
+          "log_partition": 3.0}]
+    write_csv(tmp_path / "a.csv", a, with_beta=False)
+    write_csv(tmp_path / "b.csv", b, with_beta=False)
+    with pytest.raises(SystemExit):
+        list(ta.read_rows([str(tmp_path / "a.csv")]))
+    out = ta.combine(ta.read_rows([str(tmp_path / "a.csv"), str(tmp_path / "b.csv")], single_beta=2.0),
+                     "observable-average")
+    (r,) = out
+    assert (r["beta"], r["mu"], r["twist_count"]) == (2.0, 0.0, 2)
+    assert r["n"] == 0.5 and abs(r["charge_correlation"] - 0.2) < 1e-15 and abs(r["compressibility"] - 0.4) < 1e-15
+    assert abs(r["log_partition"] - (math.log(math.exp(1) + math.exp(3)) - math.log(2))) < 1e-14
+
+
+def test_cli_and_sites_guard(tmp_path):
+    rows = [ensemble(2, 1.0, 0.0, [(0.0, 0, 1), (1.0, 1, 2)])[0]]
+    write_csv(tmp_path / "x.csv", rows)
+    with pytest.raises(SystemExit):
+        ta.main([str(tmp_path / "x.csv"), "--mode", "partition-sum", "--output", str(tmp_path / "o.csv")])
+    ta.main([str(tmp_path / "x.csv"), "--mode", "partition-sum", "--sites", "2", "--output", str(tmp_path / "o.csv")])
+    text = (tmp_path / "o.csv").read_text().splitlines()
+    assert text[0] == "beta,mu,n,charge_correlation,compressibility,twist_count,mode,log_partition"
+    assert text[1].startswith("1.0,0.0,") and ",1,partition-sum," in text[1]
