@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--block", default="8,8,0,0")
     ap.add_argument("--plain-vectors", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device copy of BYTES bytes (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
     ap.add_argument("--tile", default="auto",
                     help="H-apply configuration: auto (autotuner) or PATH,W (path 1 hop list / 2 on the fly; "
                          "W -1 automatic, 0 untiled, >0 tile width)")
@@ -98,6 +101,19 @@ def main():
         sys.exit(2)
 
     import ftlm_extended_hubbard_development_amd as ftlm
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: {n * 8} bytes read, {n * 8} bytes written", file=sys.stderr, flush=True)
+        del src, dst
+        torch.cuda.empty_cache()
 
     nup, ndn, mx, my = (int(t) for t in args.block.split(","))
     eng = ftlm.Engine(lx=4, ly=4, u=7.0)
