@@ -64,7 +64,16 @@ SIGNATURES = {
     "ftlm_set_apply_config": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "ftlm_get_apply_config": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     "ftlm_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
+    "ftlm_set_twist": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "ftlm_pack_begin": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64]),
+    "ftlm_pack_add": (C.c_int, [C.c_void_p, C.c_uint64, _lp]),
+    "ftlm_pack_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "ftlm_pack_result": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _lp]),
+    "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
+    "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
 }
+
+PACK_FULL = 1
 
 _lib = None
 
@@ -253,6 +262,66 @@ class Engine:
         names = {1: "hoplist", 2: "onthefly"}
         return ((bp.value, bw.value), bms.value,
                 {f"{names[int(cp[i])]}/W{int(cw[i])}": round(float(cm[i]), 3) for i in range(n.value)})
+
+    def set_twist(self, phix, phiy):
+        """Change the twist; the current block must be selected again."""
+        self._check(self.lib.ftlm_set_twist(self.h, phix, phiy))
+        self.model.phix = phix
+        self.model.phiy = phiy
+
+    # ---- packed blocks (one persistent launch over many blocks) ----
+    def pack_begin(self, R, steps, budget_bytes=0):
+        self._check(self.lib.ftlm_pack_begin(self.h, R, steps, budget_bytes))
+        self._pack = {"R": R, "steps": steps, "dims": []}
+
+    def pack_add(self, bseed):
+        """Add the current block; returns its index, or None when the pack is
+        full (nothing added)."""
+        idx = C.c_int64(-1)
+        rc = self.lib.ftlm_pack_add(self.h, bseed, C.byref(idx))
+        if rc == PACK_FULL:
+            return None
+        self._check(rc)
+        self._pack["dims"].append(self.dim)
+        return idx.value
+
+    def pack_run(self, width=0, wg_per_cu=0, method=0):
+        """method 0 Jacobi Ritz (bit-identical to the reference), 1 QL.
+        Returns the device-event stats of the run."""
+        self._check(self.lib.ftlm_pack_run(self.h, width, wg_per_cu, method))
+        s = FtlmStats()
+        self._check(self.lib.ftlm_last_stats(self.h, C.byref(s)))
+        return {"ms_total": s.ms_total, "n_apply": s.n_apply, "bytes_apply": s.bytes_apply}
+
+    def pack_result(self, index):
+        dims = self._pack["dims"]
+        d = dims[index] if 0 <= index < len(dims) else 1
+        n = self._pack["R"] * min(self._pack["steps"], d)
+        e = np.zeros(max(n, 1))
+        w = np.zeros(max(n, 1))
+        cnt = C.c_int64(0)
+        self._check(self.lib.ftlm_pack_result(self.h, index, _p(e, _dp), _p(w, _dp), C.byref(cnt)))
+        return e[: cnt.value], w[: cnt.value]
+
+    def pack_tune(self, tune_steps=4):
+        """Time (width, workgroups per CU) candidates; returns ((W, wg), ms, {"W/wg": ms})."""
+        bw = C.c_int32(0)
+        bg = C.c_int32(0)
+        bms = C.c_double(0)
+        n = C.c_int32(32)
+        cw = np.zeros(32, np.int32)
+        cg = np.zeros(32, np.int32)
+        cm = np.zeros(32)
+        self._check(self.lib.ftlm_pack_tune(self.h, tune_steps, C.byref(bw), C.byref(bg), C.byref(bms), C.byref(n),
+                                            _p(cw, _ip), _p(cg, _ip), _p(cm, _dp)))
+        return ((bw.value, bg.value), bms.value,
+                {f"W{int(cw[i])}/wg{int(cg[i])}": round(float(cm[i]), 3) for i in range(n.value)})
+
+    def pack_info(self):
+        n = C.c_int64(0)
+        b = C.c_uint64(0)
+        self._check(self.lib.ftlm_pack_info(self.h, C.byref(n), C.byref(b)))
+        return n.value, b.value
 
     def exact_blocks(self, ks):
         """ks: list of (mx, my) of the current sector -> list of eigenvalue arrays."""

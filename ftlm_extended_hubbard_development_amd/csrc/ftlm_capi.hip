@@ -118,6 +118,97 @@ struct BlockTables {
   BlockView view{};
 };
 
+// bump allocator over large device chunks (kept across packs)
+struct Arena {
+  std::vector<std::pair<void*, size_t>> chunks;
+  size_t cur = 0, off = 0, used = 0;
+  Arena() = default;
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  ~Arena() {
+    for (auto& ch : chunks) (void)hipFree(ch.first);
+  }
+  void reset() {
+    cur = 0;
+    off = 0;
+    used = 0;
+  }
+  void* alloc(size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    if (n == 0) n = 256;
+    while (cur < chunks.size()) {
+      if (off + n <= chunks[cur].second) {
+        void* p = static_cast<char*>(chunks[cur].first) + off;
+        off += n;
+        used += n;
+        return p;
+      }
+      ++cur;
+      off = 0;
+    }
+    const size_t sz = std::max(n, (size_t)1 << 30);
+    void* p = nullptr;
+    HIPCHECK(hipMalloc(&p, sz));
+    chunks.push_back({p, sz});
+    cur = chunks.size() - 1;
+    off = n;
+    used += n;
+    return p;
+  }
+};
+
+// grow-only pinned host buffer (hipHostMalloc)
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void* get(size_t n) {
+    if (n > bytes) {
+      if (p) HIPCHECK(hipHostFree(p));
+      p = nullptr;
+      bytes = 0;
+      HIPCHECK(hipHostMalloc(&p, n, hipHostMallocDefault));
+      bytes = n;
+    }
+    return p;
+  }
+};
+
+// blocks batched into one persistent launch (ftlm_pack_*)
+struct PackState {
+  bool open = false;
+  int R = 0, steps = 0;
+  uint64_t budget = 0;
+  Arena arena;
+  std::vector<PackEntry> entries;
+  std::vector<double> cost;       // per entry: nnz + 2 dim (rows gathered per column and step)
+  std::vector<double> alg_bytes;  // per entry: kdim * dim * R * 16 * (hbar + 2)
+  std::vector<std::pair<std::pair<uint64_t, int64_t>, const double2*>> starts;  // (seed, dim) -> start vectors
+  int64_t max_dim = 0;
+  // results of the last run
+  bool has_result = false;
+  int rcap = 0;
+  std::vector<double> eig, w;
+  std::vector<int32_t> m;
+  // tuned launch shape
+  int best_w = 0, best_wg = 0;
+  DevBuf d_entries, d_items, d_nitems, queue, fail, alpha, beta, mcount, eig_d, w_d, scratch, iscratch, slots;
+  PinnedBuf stage;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> exec_key;
+  PackState() = default;
+  PackState(const PackState&) = delete;
+  PackState& operator=(const PackState&) = delete;
+  ~PackState() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 }  // namespace
 
 struct ftlm_ctx {
@@ -165,6 +256,8 @@ struct ftlm_ctx {
   ftlm_stats stats{};
   // exact
   DevBuf ex_mats, ex_work, ex_eig, ex_moff, ex_dims, ex_eoff, ex_views;
+  // packed blocks
+  PackState pack;
 
   ~ftlm_ctx() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -172,6 +265,25 @@ struct ftlm_ctx {
 };
 
 namespace {
+
+// hopping directions with non-zero amplitude, and their amplitudes
+// -t * twist (main.cpp:544-577, 683-687) for fermion sign +1
+void set_amplitudes(ftlm_ctx* c) {
+  const ftlm_model& m = c->model;
+  LatticeConst& g = c->g;
+  const double ax = 2.0 * kPi * m.phix / (double)m.lx;
+  const double ay = 2.0 * kPi * m.phiy / (double)m.ly;
+  const double ang[8] = {ax, -ax, ay, -ay, ax + ay, -ax + ay, ax - ay, -ax - ay};
+  g.ndir = 0;
+  for (int d = 0; d < 8; ++d) {
+    const double t = d < 2 ? m.tx : (d < 4 ? m.ty : m.tp);
+    const double a = -t * 1.0;
+    const double cr = std::cos(ang[d]), ci = std::sin(ang[d]);
+    c->amp_re[d] = a * cr;
+    c->amp_im[d] = a * ci;
+    if (t != 0.0) g.dirs[g.ndir++] = (int8_t)d;
+  }
+}
 
 void build_lattice(ftlm_ctx* c) {
   const ftlm_model& m = c->model;
@@ -214,20 +326,7 @@ void build_lattice(ftlm_ctx* c) {
   }
   g.U = m.u;
   g.V = m.v;
-  // hopping directions with non-zero amplitude, and their amplitudes
-  // -t * twist (main.cpp:544-577, 683-687) for fermion sign +1
-  const double ax = 2.0 * kPi * m.phix / (double)lx;
-  const double ay = 2.0 * kPi * m.phiy / (double)ly;
-  const double ang[8] = {ax, -ax, ay, -ay, ax + ay, -ax + ay, ax - ay, -ax - ay};
-  g.ndir = 0;
-  for (int d = 0; d < 8; ++d) {
-    const double t = d < 2 ? m.tx : (d < 4 ? m.ty : m.tp);
-    const double a = -t * 1.0;
-    const double cr = std::cos(ang[d]), ci = std::sin(ang[d]);
-    c->amp_re[d] = a * cr;
-    c->amp_im[d] = a * ci;
-    if (t != 0.0) g.dirs[g.ndir++] = (int8_t)d;
-  }
+  set_amplitudes(c);
 }
 
 void upload_ctx_tables(ftlm_ctx* c) {
@@ -821,6 +920,273 @@ void check_R(int R) {
   if (R < 1 || R > 1024) throw Error(FTLM_ERR_LIMIT, "R must be in [1, 1024]");
 }
 
+// ---- packed blocks ---------------------------------------------------------
+constexpr uint64_t kPackDefaultBudget = (uint64_t)64 << 30;
+constexpr size_t kPackSlotLimit = (size_t)32 << 30;  // bytes of per-workgroup vectors
+
+size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int pack_width(const PackState& p, int width) {
+  int W = width > 0 ? width : p.best_w;
+  if (W <= 0) W = p.R >= 16 ? 16 : (p.R > 8 ? 16 : (p.R > 4 ? 8 : 4));
+  if (W != 4 && W != 8 && W != 16 && W != 32 && W != 64) throw Error(FTLM_ERR_ARG, "pack width must be 4, 8, 16, 32 or 64");
+  return W;
+}
+
+// items (entry, column group) in decreasing cost; ties keep entry order
+std::vector<PackItem> pack_items(const PackState& p, int W) {
+  std::vector<PackItem> items;
+  for (size_t e = 0; e < p.entries.size(); ++e)
+    for (int c0 = 0; c0 < p.R; c0 += W) items.push_back({(int32_t)e, c0, std::min(W, p.R - c0), 0});
+  std::stable_sort(items.begin(), items.end(), [&](const PackItem& a, const PackItem& b) {
+    return p.cost[a.entry] * a.w > p.cost[b.entry] * b.w;
+  });
+  return items;
+}
+
+// Launch state for one run: uploads the tables and returns the kernel args.
+PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax) {
+  PackState& p = c->pack;
+  const std::vector<PackItem> items = pack_items(p, W);
+  const int ncols = (int)p.entries.size() * p.R;
+  const int rcap = std::max(p.rcap, (ncols + 1023) / 1024 * 1024);
+  p.rcap = rcap;
+  const int64_t stride = std::max<int64_t>(1, p.max_dim) * W;
+  const size_t per_wg = (size_t)3 * stride * 16;
+  int g = std::max(1, std::min(*grid, (int)std::min<size_t>(kPackSlotLimit / per_wg, (size_t)1 << 20)));
+  g = std::min<int64_t>(g, (int64_t)items.size());
+  *grid = g;
+  PackArgs a{};
+  PackEntry* de = p.d_entries.as<PackEntry>(p.entries.size());
+  PackItem* di = p.d_items.as<PackItem>(items.size());
+  int32_t* dn = p.d_nitems.as<int32_t>(1);
+  HIPCHECK(hipMemcpyAsync(de, p.entries.data(), p.entries.size() * sizeof(PackEntry), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipMemcpyAsync(di, items.data(), items.size() * sizeof(PackItem), hipMemcpyHostToDevice, c->stream));
+  const int32_t nitems = (int32_t)items.size();
+  HIPCHECK(hipMemcpyAsync(dn, &nitems, 4, hipMemcpyHostToDevice, c->stream));
+  a.entries = de;
+  a.items = di;
+  a.n_items = dn;
+  a.queue = p.queue.as<unsigned>(4);
+  a.fail = p.fail.as<int32_t>(4);
+  a.slots = p.slots.as<double2>((size_t)g * 3 * stride);
+  a.slot_stride = stride;
+  a.rcap = rcap;
+  a.kmax = kmax;
+  a.alpha = p.alpha.as<double>((size_t)p.steps * rcap);
+  a.beta = p.beta.as<double>((size_t)p.steps * rcap);
+  a.mcount = p.mcount.as<int32_t>(rcap);
+  sync(c);  // the host item vector goes out of scope
+  return a;
+}
+
+void pack_ritz(ftlm_ctx* c, const PackArgs& a, int method) {
+  PackState& p = c->pack;
+  const int rcap = a.rcap, steps = p.steps;
+  double* eig = p.eig_d.as<double>((size_t)rcap * steps);
+  double* w = p.w_d.as<double>((size_t)rcap * steps);
+  if (method == 0) {
+    const size_t words = (size_t)steps * (steps + 1) / 2 + 3 * (size_t)steps;
+    double* scratch = p.scratch.as<double>(words * 8 <= 60 * 1024 ? 1 : words * rcap);
+    launch_ritz(rcap, steps, a.alpha, a.beta, a.mcount, p.R, eig, w, scratch, c->stream);
+  } else {
+    double* scratch = p.scratch.as<double>((size_t)3 * steps * rcap);
+    int32_t* isc = p.iscratch.as<int32_t>((size_t)steps * rcap + 1);
+    launch_ritz_ql(rcap, steps, a.alpha, a.beta, a.mcount, p.R, eig, w, scratch, isc, a.fail, c->stream);
+  }
+}
+
+// One run of the whole pack through a HIP graph (reset, persistent Lanczos,
+// Ritz), instantiated once per launch shape and buffer set and replayed for
+// later packs with the same shape; returns the graph's wall time in ms.
+double pack_run(ftlm_ctx* c, int width, int wg_per_cu, int method) {
+  PackState& p = c->pack;
+  if (!p.open) throw Error(FTLM_ERR_STATE, "no pack (call ftlm_pack_begin first)");
+  if (method != 0 && method != 1) throw Error(FTLM_ERR_ARG, "method must be 0 (Jacobi) or 1 (QL)");
+  p.has_result = false;
+  const int W = pack_width(p, width);
+  if (p.entries.empty()) {
+    p.has_result = true;
+    return 0.0;
+  }
+  int wg = wg_per_cu > 0 ? wg_per_cu : (p.best_wg > 0 ? p.best_wg : 2);
+  int grid = c->cus * wg;
+  const PackArgs a = pack_args(c, W, &grid, 0);
+  // the Ritz buffers must exist before capture (no allocation inside it)
+  const int rcap = a.rcap, steps = p.steps;
+  p.eig_d.as<double>((size_t)rcap * steps);
+  p.w_d.as<double>((size_t)rcap * steps);
+  const size_t words = (size_t)steps * (steps + 1) / 2 + 3 * (size_t)steps;
+  if (method == 0) {
+    p.scratch.as<double>(words * 8 <= 60 * 1024 ? 1 : words * rcap);
+  } else {
+    p.scratch.as<double>((size_t)3 * steps * rcap);
+    p.iscratch.as<int32_t>((size_t)steps * rcap + 1);
+  }
+  std::vector<uint64_t> key = {(uint64_t)W, (uint64_t)grid, (uint64_t)rcap, (uint64_t)steps, (uint64_t)method,
+                               (uint64_t)p.R, (uint64_t)a.slot_stride, (uint64_t)(uintptr_t)a.entries,
+                               (uint64_t)(uintptr_t)a.items, (uint64_t)(uintptr_t)a.slots,
+                               (uint64_t)(uintptr_t)a.alpha, (uint64_t)(uintptr_t)a.beta,
+                               (uint64_t)(uintptr_t)a.mcount, (uint64_t)(uintptr_t)p.eig_d.p,
+                               (uint64_t)(uintptr_t)p.w_d.p, (uint64_t)(uintptr_t)p.scratch.p,
+                               (uint64_t)(uintptr_t)p.iscratch.p};
+  if (!p.exec || key != p.exec_key) {
+    if (p.exec) {
+      HIPCHECK(hipGraphExecDestroy(p.exec));
+      p.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_pack_reset(a, rcap, c->stream);
+    launch_pack_lanczos(a, W, grid, c->stream);
+    pack_ritz(c, a, method);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (ce != hipSuccess) throw Error(FTLM_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&p.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      p.exec = nullptr;
+      throw Error(FTLM_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+    }
+    p.exec_key = key;
+  }
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  HIPCHECK(hipEventRecord(e0, c->stream));
+  HIPCHECK(hipGraphLaunch(p.exec, c->stream));
+  HIPCHECK(hipEventRecord(e1, c->stream));
+  sync(c);
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int32_t nf = 0;
+  if (method == 1) {
+    HIPCHECK(hipMemcpy(&nf, a.fail, 4, hipMemcpyDeviceToHost));
+    if (nf) throw Error(FTLM_ERR_DEVICE, "tridiagonal QL did not converge");
+  }
+  const int ncols = (int)p.entries.size() * p.R;
+  p.eig.resize((size_t)ncols * steps);
+  p.w.resize((size_t)ncols * steps);
+  p.m.resize(ncols);
+  HIPCHECK(hipMemcpy(p.eig.data(), p.eig_d.p, p.eig.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(p.w.data(), p.w_d.p, p.w.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(p.m.data(), a.mcount, 4 * (size_t)ncols, hipMemcpyDeviceToHost));
+  p.has_result = true;
+  c->stats = ftlm_stats{};
+  c->stats.ms_total = ms;
+  c->stats.ms_apply = ms;
+  c->stats.n_apply = 0;
+  for (size_t e = 0; e < p.entries.size(); ++e) {
+    c->stats.n_apply += p.entries[e].kdim;
+    c->stats.bytes_apply += p.alg_bytes[e];
+  }
+  return ms;
+}
+
+// Timing of the persistent kernel alone over `kmax` steps (tuning).
+double pack_time(ftlm_ctx* c, int W, int wg_per_cu, int kmax) {
+  int grid = c->cus * wg_per_cu;
+  const PackArgs a = pack_args(c, W, &grid, kmax);
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  launch_pack_reset(a, a.rcap, c->stream);
+  HIPCHECK(hipEventRecord(e0, c->stream));
+  launch_pack_lanczos(a, W, grid, c->stream);
+  HIPCHECK(hipEventRecord(e1, c->stream));
+  HIPCHECK(hipGetLastError());
+  sync(c);
+  float ms = 0.f;
+  HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->pack.has_result = false;
+  return ms;
+}
+
+// Add the current block: hop list into the pack's arena, start vectors
+// generated on the host (shared by entries of equal seed and dimension, e.g.
+// the twists of one block). Returns false when the budget is exceeded.
+bool pack_add(ftlm_ctx* c, uint64_t bseed, int64_t* index) {
+  PackState& p = c->pack;
+  if (!p.open) throw Error(FTLM_ERR_STATE, "no pack (call ftlm_pack_begin first)");
+  need_block(c);
+  BlockTables& b = c->cur;
+  const int64_t n = b.dim;
+  if (n <= 0) throw Error(FTLM_ERR_ARG, "empty block");
+  if ((int64_t)(p.entries.size() + 1) * p.R > INT32_MAX / 2) throw Error(FTLM_ERR_LIMIT, "too many pack columns");
+  hipStream_t s = c->stream;
+  int64_t* cnt = c->hl_cnt.as<int64_t>(n + 1);
+  int64_t* scnt = c->hl_scnt.as<int64_t>(n + 1);
+  int64_t* rp_tmp = c->hl_rowptr.as<int64_t>(n + 1);
+  int64_t* srp = c->hl_srowptr.as<int64_t>(n + 1);
+  c->hl_key = -1;  // the ctx hop-list buffers are reused here
+  launch_hops_count(gdev(c), b.view, cnt, scnt, s);
+  launch_prefix64(cnt, n, rp_tmp, s);
+  launch_prefix64(scnt, n, srp, s);
+  HIPCHECK(hipGetLastError());
+  int64_t tot[2];
+  HIPCHECK(hipMemcpyAsync(&tot[0], rp_tmp + n, 8, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(&tot[1], srp + n, 8, hipMemcpyDeviceToHost, s));
+  sync(c);
+  const int64_t nnz = tot[0], nsp = tot[1];
+  if (nnz > INT32_MAX) throw Error(FTLM_ERR_LIMIT, "hop list too large for a pack entry");
+  const int L = c->g.L;
+  const double2* start = nullptr;
+  for (auto& st : p.starts)
+    if (st.first.first == bseed && st.first.second == n) start = st.second;
+  const size_t need = al256(8 * (size_t)(n + 1)) + 2 * al256(4 * (size_t)(nnz + 1)) +
+                      al256(16 * (size_t)(nsp + 1)) + al256(8 * (size_t)n) + al256(128 * (size_t)L) +
+                      (start ? 0 : al256((size_t)16 * n * p.R));
+  if (!p.entries.empty() && p.arena.used + need > p.budget) return false;
+  int64_t* rp = static_cast<int64_t*>(p.arena.alloc(8 * (size_t)(n + 1)));
+  int32_t* col = static_cast<int32_t*>(p.arena.alloc(4 * (size_t)(nnz + 1)));
+  uint32_t* code = static_cast<uint32_t*>(p.arena.alloc(4 * (size_t)(nnz + 1)));
+  double* spec = static_cast<double*>(p.arena.alloc(16 * (size_t)(nsp + 1)));
+  double* diag = static_cast<double*>(p.arena.alloc(8 * (size_t)n));
+  double* ptab = static_cast<double*>(p.arena.alloc(128 * (size_t)L));
+  HIPCHECK(hipMemcpyAsync(rp, rp_tmp, 8 * (size_t)(n + 1), hipMemcpyDeviceToDevice, s));
+  HIPCHECK(hipMemcpyAsync(ptab, b.view.ptab, 128 * (size_t)L, hipMemcpyDeviceToDevice, s));
+  launch_hops_fill(gdev(c), b.view, rp, srp, col, code, spec, diag, s);
+  HIPCHECK(hipGetLastError());
+  if (!start) {
+    double2* st = static_cast<double2*>(p.arena.alloc((size_t)16 * n * p.R));
+    const int64_t fit = std::max<int64_t>(1, (int64_t)(1ll << 30) / std::max<int64_t>(1, n * 16));
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)p.R, fit));
+    double* host = static_cast<double*>(p.stage.get((size_t)chunk * n * 16));
+    for (int r0 = 0; r0 < p.R; r0 += chunk) {
+      const int cnt_r = std::min(chunk, p.R - r0);
+      sync(c);  // staging reuse
+      gen_starts(bseed, r0, cnt_r, n, host);
+      HIPCHECK(hipMemcpyAsync(st + (int64_t)r0 * n, host, (size_t)cnt_r * n * 16, hipMemcpyHostToDevice, s));
+    }
+    start = st;
+    p.starts.push_back({{bseed, n}, st});
+  }
+  sync(c);
+  PackEntry e{};
+  e.row_ptr = rp;
+  e.col = col;
+  e.code = code;
+  e.spec = reinterpret_cast<const double2*>(spec);
+  e.diag = diag;
+  e.ptab = reinterpret_cast<const double2*>(ptab);
+  e.start = start;
+  e.dim = n;
+  e.kdim = (int32_t)std::min<int64_t>(p.steps, n);
+  e.col0 = (int32_t)(p.entries.size() * p.R);
+  if (index) *index = (int64_t)p.entries.size();
+  p.entries.push_back(e);
+  p.cost.push_back((double)nnz + 2.0 * (double)n);
+  p.alg_bytes.push_back((double)e.kdim * (double)n * p.R * 16.0 * (hbar(c) + 2.0));
+  p.max_dim = std::max(p.max_dim, n);
+  p.has_result = false;
+  return true;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -1172,6 +1538,124 @@ int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t*
     for (int i = 0; i <= nblk; ++i) offsets[i] = eoff[i];
     for (int64_t i = 0; i < eoff[nblk]; ++i)
       if (std::isnan(eig_out[i])) throw Error(FTLM_ERR_DEVICE, "exact eigensolver did not converge");
+  });
+}
+
+int ftlm_set_twist(ftlm_ctx* ctx, double phix, double phiy) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    if (!std::isfinite(phix) || !std::isfinite(phiy)) throw Error(FTLM_ERR_ARG, "twist must be finite");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->model.phix = phix;
+    ctx->model.phiy = phiy;
+    set_amplitudes(ctx);
+    HIPCHECK(hipMemcpy(ctx->g_dev.p, &ctx->g, sizeof(LatticeConst), hipMemcpyHostToDevice));
+    ctx->cur.mx = ctx->cur.my = -1;
+    ctx->prepared_R = 0;
+    ++ctx->block_gen;
+  });
+}
+
+int ftlm_pack_begin(ftlm_ctx* ctx, int R, int steps, uint64_t budget_bytes) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    check_R(R);
+    if (steps < 1) throw Error(FTLM_ERR_ARG, "steps must be positive");
+    HIPCHECK(hipSetDevice(ctx->device));
+    PackState& p = ctx->pack;
+    p.open = true;
+    p.R = R;
+    p.steps = steps;
+    p.budget = budget_bytes ? budget_bytes : kPackDefaultBudget;
+    p.arena.reset();
+    p.entries.clear();
+    p.cost.clear();
+    p.alg_bytes.clear();
+    p.starts.clear();
+    p.max_dim = 0;
+    p.has_result = false;
+  });
+}
+
+int ftlm_pack_add(ftlm_ctx* ctx, uint64_t block_seed, int64_t* index) {
+  bool added = true;
+  const int rc = guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    added = pack_add(ctx, block_seed, index);
+  });
+  if (rc != FTLM_OK) return rc;
+  return added ? FTLM_OK : FTLM_PACK_FULL;
+}
+
+int ftlm_pack_run(ftlm_ctx* ctx, int width, int wg_per_cu, int method) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    (void)pack_run(ctx, width, wg_per_cu, method);
+  });
+}
+
+int ftlm_pack_result(ftlm_ctx* ctx, int64_t index, double* eig, double* w, int64_t* count) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    const PackState& p = ctx->pack;
+    if (!p.has_result) throw Error(FTLM_ERR_STATE, "pack has not been run");
+    if (index < 0 || index >= (int64_t)p.entries.size()) throw Error(FTLM_ERR_ARG, "pack index out of range");
+    int64_t k = 0;
+    for (int r = 0; r < p.R; ++r) {
+      const int64_t col = index * p.R + r;
+      for (int j = 0; j < p.m[col]; ++j) {
+        if (eig) eig[k] = p.eig[(size_t)col * p.steps + j];
+        if (w) w[k] = p.w[(size_t)col * p.steps + j];
+        ++k;
+      }
+    }
+    if (count) *count = k;
+  });
+}
+
+int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_per_cu, double* best_ms,
+                   int* ncand, int* cand_width, int* cand_wg, double* cand_ms) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    PackState& p = ctx->pack;
+    if (!p.open || p.entries.empty()) throw Error(FTLM_ERR_STATE, "empty pack");
+    const int cap = ncand ? *ncand : 0;
+    int n = 0, bw = 0, bg = 0;
+    double bms = 0.0;
+    for (int W : {8, 16, 32, 64}) {
+      if (W > 16 && W >= 2 * p.R) continue;
+      for (int wg : {1, 2, 4}) {
+        const double ms = pack_time(ctx, W, wg, tune_steps > 0 ? tune_steps : 4);
+        if (n < cap) {
+          if (cand_width) cand_width[n] = W;
+          if (cand_wg) cand_wg[n] = wg;
+          if (cand_ms) cand_ms[n] = ms;
+        }
+        ++n;
+        if (bw == 0 || ms < bms) {
+          bw = W;
+          bg = wg;
+          bms = ms;
+        }
+      }
+    }
+    p.best_w = bw;
+    p.best_wg = bg;
+    if (best_width) *best_width = bw;
+    if (best_wg_per_cu) *best_wg_per_cu = bg;
+    if (best_ms) *best_ms = bms;
+    if (ncand) *ncand = std::min(n, cap);
+  });
+}
+
+int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    if (entries) *entries = (int64_t)ctx->pack.entries.size();
+    if (bytes) *bytes = (uint64_t)ctx->pack.arena.used;
   });
 }
 

@@ -117,6 +117,43 @@ void launch_ritz_ql(int R, int steps, const double* alpha, const double* beta, c
 void launch_ritz(int R, int steps, const double* alpha, const double* beta, const int32_t* mcount, int wdiv,
                  double* eig, double* w, double* scratch, hipStream_t s);
 
+// ---- packed Lanczos (ftlm_pack.hip) ----
+// One block of a pack: its hop list, diagonal, coefficient tables and
+// sample-major start vectors [R][dim]; its R columns are col0..col0+R-1 of
+// the pack's alpha/beta/mcount arrays.
+struct PackEntry {
+  const int64_t* row_ptr;
+  const int32_t* col;
+  const uint32_t* code;
+  const double2* spec;
+  const double* diag;
+  const double2* ptab;
+  const double2* start;
+  int64_t dim;
+  int32_t kdim;  // min(steps, dim)
+  int32_t col0;
+};
+// A work item: columns c0..c0+w-1 of one entry.
+struct PackItem {
+  int32_t entry, c0, w, pad;
+};
+struct PackArgs {
+  const PackEntry* entries;
+  const PackItem* items;      // in the order they are taken
+  const int32_t* n_items;     // device scalar (read by the kernel: graph replays see new packs)
+  unsigned* queue;            // next item (reset by launch_pack_reset)
+  int32_t* fail;              // QL non-convergence count (reset likewise)
+  double2* slots;             // per workgroup: 3 vectors of slot_stride
+  int64_t slot_stride;        // >= max dim * W
+  int32_t rcap;               // column capacity (row stride of alpha/beta)
+  int32_t kmax;               // > 0: stop after kmax steps (tuning runs)
+  double* alpha;              // [steps][rcap]
+  double* beta;               // [steps][rcap]
+  int32_t* mcount;            // [rcap]
+};
+void launch_pack_reset(const PackArgs& a, int rcap, hipStream_t s);
+void launch_pack_lanczos(const PackArgs& a, int W, int grid, hipStream_t s);
+
 // ---- exact (ftlm_exact.hip) ----
 void launch_herm_eig(const int64_t* moff, const int64_t* dims, int nblk, double* mats, double* work,
                      double* eig_out, const int64_t* eoff, hipStream_t s);

@@ -140,6 +140,45 @@ int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W);
 int ftlm_autotune(ftlm_ctx* ctx, int R, int* best_path, int* best_w, double* best_ms, int* ncand, int* cand_path,
                   int* cand_w, double* cand_ms);
 
+/* Twist of the context (Params phix/phiy, main.cpp:544-577): blocks selected
+ * afterwards use the new hop phases. Sector tables do not depend on the twist
+ * and are kept; the current block must be selected again (ftlm_block). */
+int ftlm_set_twist(ftlm_ctx* ctx, double phix, double phiy);
+
+/* Packed FTLM over many blocks of any sector and twist (no reference
+ * counterpart: the block loop main.cpp:1519-1581 runs run_ftlm_block block by
+ * block; here the uneven small blocks of a sweep are batched into one
+ * persistent launch, replayed through a HIP graph).
+ *   ftlm_pack_begin   empties the pack: R samples and `steps` Lanczos steps
+ *                     per block, device memory budget in bytes (0: 64 GiB).
+ *   ftlm_pack_add     adds the current block with its block seed (hop list
+ *                     built on the device, start vectors generated on the
+ *                     host as run_ftlm_block draws them); *index receives its
+ *                     position. Returns FTLM_PACK_FULL, adding nothing, when
+ *                     the block does not fit the budget: run the pack first.
+ *   ftlm_pack_run     runs every block. Work items are (block, `width`
+ *                     columns), width 4/8/16/32/64 or 0 (tuned); `wg_per_cu`
+ *                     workgroups per CU (0: tuned). Each workgroup runs all
+ *                     Lanczos steps of its item with row-order reductions, so
+ *                     alpha/beta are bit-identical to the reference for every
+ *                     width; method 0 = Jacobi Ritz (bit-identical to
+ *                     run_ftlm_block), 1 = implicit QL.
+ *   ftlm_pack_result  Ritz values and weights of entry `index`, as
+ *                     ftlm_ftlm_block returns them.
+ *   ftlm_pack_tune    times the (width, wg_per_cu) candidates on the current
+ *                     pack over `tune_steps` steps and keeps the fastest for
+ *                     later runs with 0 arguments.
+ */
+#define FTLM_PACK_FULL 1
+int ftlm_pack_begin(ftlm_ctx* ctx, int R, int steps, uint64_t budget_bytes);
+int ftlm_pack_add(ftlm_ctx* ctx, uint64_t block_seed, int64_t* index);
+int ftlm_pack_run(ftlm_ctx* ctx, int width, int wg_per_cu, int method);
+int ftlm_pack_result(ftlm_ctx* ctx, int64_t index, double* eig, double* w, int64_t* count);
+int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_per_cu, double* best_ms,
+                   int* ncand, int* cand_width, int* cand_wg, double* cand_ms);
+/* entries in the pack and device bytes they hold */
+int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes);
+
 /* diagonalize_block_exact for blocks (mx[i], my[i]) of the current sector,
  * batched in one launch: eig_out receives the ascending eigenvalues of each
  * block back to back, offsets[i]..offsets[i+1]. */

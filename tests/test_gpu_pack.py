@@ -1,0 +1,195 @@
+"""Packed FTLM (ftlm_pack_*: many blocks of different sectors and twists in
+one persistent launch, replayed through a HIP graph) against the oracle.
+
+Each workgroup runs all Lanczos steps of its (block, column group) item with
+row-order reductions, so every entry must equal the reference's
+run_ftlm_block bit for bit (the oracle is bit-identical to it) for every
+column-group width and grid size. The QL Ritz method is compared through the
+block's thermodynamic sums (tolerance 1e-10 relative)."""
+import numpy as np
+import pytest
+
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+MODEL_3x3 = dict(lx=3, ly=3, tp=-0.25, u=6.0, v=1.0)
+TWISTS = [(0.3, 0.7), (0.0, 0.5)]
+# (nup, ndn, mx, my): dims from a handful of rows (fewer than the steps) to 1764
+BLOCKS_3x3 = [(3, 3, 0, 0), (4, 3, 1, 2), (2, 1, 1, 1), (1, 1, 0, 0), (4, 4, 2, 1), (0, 2, 1, 0), (5, 4, 0, 2)]
+MODEL_4x2 = dict(lx=4, ly=2, u=8.0)
+BLOCKS_4x2 = [(4, 4, 0, 0), (3, 4, 1, 1), (2, 2, 2, 0), (4, 4, 2, 1), (3, 3, 3, 0)]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pkg():
+    import ftlm_extended_hubbard_development_amd as p
+
+    p.load_library()
+    return p
+
+
+_oracle = {}
+
+
+def oracle_result(orc, md, case, R, steps, seed=12345):
+    key = (tuple(sorted(md.items())), case, R, steps, seed)
+    if key not in _oracle:
+        blk = orc.block(oracle_lib.model(**md), *case)
+        bs = orc.block_seed(seed, *case)
+        _oracle[key] = orc.ftlm_block(blk, R, steps, bs, threads=8)
+    return _oracle[key]
+
+
+def fill_pack(pkg, eng, entries, R, steps, seed=12345, budget=0):
+    """entries: list of (twist, case). Returns the pack indices."""
+    eng.pack_begin(R, steps, budget)
+    idx = []
+    tw = None
+    for twist, case in entries:
+        if twist != tw:
+            eng.set_twist(*twist)  # sector tables are kept; blocks are re-selected below
+            tw = twist
+        if (eng.nup, eng.ndn) != case[:2]:
+            eng.sector(*case[:2])
+        eng.block(*case[2:])
+        i = eng.pack_add(pkg.block_seed(seed, *case))
+        if i is None:
+            break
+        idx.append(i)
+    return idx
+
+
+def mixed_entries():
+    return [(tw, c) for tw in TWISTS for c in BLOCKS_3x3]
+
+
+@pytest.fixture(scope="module")
+def eng3(pkg):
+    e = pkg.Engine(**MODEL_3x3, phix=TWISTS[0][0], phiy=TWISTS[0][1])
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("W,wg", [(4, 1), (8, 2), (16, 1), (16, 2), (32, 2), (64, 1)])
+@pytest.mark.parametrize("R,steps", [(8, 40), (20, 30)])
+def test_pack_bitexact_sectors_and_twists(pkg, orc, eng3, W, wg, R, steps):
+    ents = mixed_entries()
+    say(f"pack 3x3 entries={len(ents)} R={R} steps={steps} W={W} wg={wg}")
+    idx = fill_pack(pkg, eng3, ents, R, steps)
+    assert idx == list(range(len(ents)))
+    eng3.pack_run(W, wg, method=0)
+    for i, (tw, case) in zip(idx, ents):
+        md = dict(MODEL_3x3, phix=tw[0], phiy=tw[1])
+        e, w = eng3.pack_result(i)
+        oe, ow = oracle_result(orc, md, case, R, steps)
+        assert np.array_equal(e, oe) and np.array_equal(w, ow), (tw, case)
+
+
+def test_pack_4x2_double_bonds(pkg, orc):
+    """4x2: ly = 2 doubles the y bonds and hops (kept as separate hops)."""
+    eng = pkg.Engine(**MODEL_4x2)
+    try:
+        ents = [((1.0, 1.0), c) for c in BLOCKS_4x2]
+        idx = fill_pack(pkg, eng, ents, 32, 120)
+        eng.pack_run(16, 2, method=0)
+        for i, (_, case) in zip(idx, ents):
+            e, w = eng.pack_result(i)
+            oe, ow = oracle_result(orc, MODEL_4x2, case, 32, 120)
+            assert np.array_equal(e, oe) and np.array_equal(w, ow), case
+    finally:
+        eng.close()
+
+
+def _thermo(e, w, pref, N, beta, mu):
+    sh = e - mu * N
+    m = sh.min()
+    return np.sum(pref * w * np.exp(-beta * (sh - m))), m
+
+
+def test_pack_ql_thermo(pkg, orc, eng3):
+    ents = mixed_entries()
+    idx = fill_pack(pkg, eng3, ents, 8, 40)
+    eng3.pack_run(16, 2, method=1)
+    for i, (tw, case) in zip(idx, ents):
+        md = dict(MODEL_3x3, phix=tw[0], phiy=tw[1])
+        e, w = eng3.pack_result(i)
+        oe, ow = oracle_result(orc, md, case, 8, 40)
+        assert e.shape == oe.shape and abs(w.sum() - 1.0) < 1e-12
+        N = case[0] + case[1]
+        for mu in np.linspace(-3, 6, 7):
+            z, m = _thermo(e, w, 1.0, N, 10.0, mu)
+            zo, mo = _thermo(oe, ow, 1.0, N, 10.0, mu)
+            assert abs(m - mo) < 1e-10 and abs(z - zo) <= 1e-10 * abs(zo), (case, mu)
+
+
+def test_pack_graph_replay(pkg, orc, eng3):
+    """The same graph is replayed several times, with eager engine work and
+    synchronisations in between; every replay gives the same bits. A second
+    pack of the same shape (different blocks) reuses the graph."""
+    ents = mixed_entries()
+    idx = fill_pack(pkg, eng3, ents, 8, 40)
+    ref = None
+    for rep in range(3):
+        eng3.pack_run(16, 2, method=0)
+        got = [eng3.pack_result(i) for i in idx]
+        if ref is None:
+            ref = got
+        for (e, w), (e0, w0) in zip(got, ref):
+            assert np.array_equal(e, e0) and np.array_equal(w, w0)
+        # eager work on the current block between replays
+        x = np.random.default_rng(rep).standard_normal((eng3.dim, 2)) + 0j
+        eng3.apply(x)
+    for i, (tw, case) in zip(idx, ents):
+        oe, ow = oracle_result(orc, dict(MODEL_3x3, phix=tw[0], phiy=tw[1]), case, 8, 40)
+        assert np.array_equal(ref[i][0], oe) and np.array_equal(ref[i][1], ow)
+    # same launch shape, other blocks (reversed order, other seed)
+    ents2 = list(reversed(ents))
+    idx2 = fill_pack(pkg, eng3, ents2, 8, 40, seed=777)
+    eng3.pack_run(16, 2, method=0)
+    for i, (tw, case) in zip(idx2, ents2):
+        e, w = eng3.pack_result(i)
+        oe, ow = oracle_result(orc, dict(MODEL_3x3, phix=tw[0], phiy=tw[1]), case, 8, 40, seed=777)
+        assert np.array_equal(e, oe) and np.array_equal(w, ow)
+
+
+def test_pack_budget_full(pkg, orc, eng3):
+    """A pack refuses the entry that would exceed its budget; the entries
+    already in it run normally."""
+    ents = mixed_entries()
+    idx = fill_pack(pkg, eng3, ents, 8, 40, budget=3 << 20)
+    assert 0 < len(idx) < len(ents)
+    n, used = eng3.pack_info()
+    assert n == len(idx) and used <= 3 << 20
+    eng3.pack_run(8, 1, method=0)
+    for i in idx:
+        tw, case = ents[i]
+        e, w = eng3.pack_result(i)
+        oe, ow = oracle_result(orc, dict(MODEL_3x3, phix=tw[0], phiy=tw[1]), case, 8, 40)
+        assert np.array_equal(e, oe) and np.array_equal(w, ow)
+
+
+def test_pack_tune(pkg, orc, eng3):
+    ents = mixed_entries()
+    idx = fill_pack(pkg, eng3, ents, 20, 30)
+    (bw, bg), ms, cand = eng3.pack_tune(3)
+    say(f"pack tune: {cand} -> W{bw}/wg{bg}")
+    assert bw in (8, 16, 32, 64) and bg in (1, 2, 4) and ms > 0 and len(cand) >= 6
+    eng3.pack_run(0, 0, method=0)
+    for i, (tw, case) in zip(idx, ents):
+        e, w = eng3.pack_result(i)
+        oe, ow = oracle_result(orc, dict(MODEL_3x3, phix=tw[0], phiy=tw[1]), case, 20, 30)
+        assert np.array_equal(e, oe) and np.array_equal(w, ow)
+
+
+def test_pack_errors(pkg, eng3):
+    eng3.pack_begin(8, 40)
+    with pytest.raises(pkg.FtlmError):
+        eng3.pack_result(0)
+    eng3.pack_run()  # empty pack: nothing to do
+    with pytest.raises(pkg.FtlmError):
+        eng3.pack_run(12, 1)  # width must be a power of two in [4, 64]
