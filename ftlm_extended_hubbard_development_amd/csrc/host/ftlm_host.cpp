@@ -96,6 +96,13 @@ void print_help(const char* argv0) {
             << "                        (serial sums in row order, bit-identical to the CPU code)\n"
             << "  --per-vector          run each random vector alone (plain per-vector GPU FTLM)\n"
             << "  --timing-json PATH    write per-block device timings\n"
+            << "  --pack-max-dim N      FTLM blocks up to N rows run packed in persistent launches\n"
+            << "                        (default 131072; 0 runs every block on its own)\n"
+            << "  --pack-width W        columns per packed work item: 4, 8, 16, 32, 64 (default: tuned)\n"
+            << "  --pack-wg N           packed workgroups per CU (default: tuned)\n"
+            << "  --pack-budget-gb G    device memory per pack in GiB (default 64)\n"
+            << "  --twist-grid N        run the twists phix, phiy in {0, 1/N, ..., (N-1)/N} together;\n"
+            << "                        one CSV per twist, OUTPUT with .phixA_phiyB before the extension\n"
             << "  --help                show this message\n";
 }
 
@@ -146,6 +153,11 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
       else die("--reduction must be tree or serial");
     } else if (arg == "--per-vector") p.per_vector = true;
     else if (arg == "--timing-json") p.timing_json = next(arg);
+    else if (arg == "--pack-max-dim") p.pack_max_dim = parse_as<std::int64_t>(next(arg));
+    else if (arg == "--pack-width") p.pack_width = parse_as<int>(next(arg));
+    else if (arg == "--pack-wg") p.pack_wg = parse_as<int>(next(arg));
+    else if (arg == "--pack-budget-gb") p.pack_budget_gb = parse_as<double>(next(arg));
+    else if (arg == "--twist-grid") p.twist_grid = parse_as<int>(next(arg));
     else die("Unknown argument: " + arg);
   }
   if (p.lx <= 0 || p.ly <= 0) die("Lattice dimensions must be positive.");
@@ -157,6 +169,12 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
   if (p.samples <= 0 || p.steps <= 0) die("--samples and --lanczos-steps must be positive.");
   if (p.threads < 0) die("--threads must be non-negative.");
   if (p.mu_count < 2) die("--mu-count must be at least 2.");
+  if (p.pack_max_dim < 0) die("--pack-max-dim must be non-negative.");
+  if (p.pack_width != 0 && p.pack_width != 4 && p.pack_width != 8 && p.pack_width != 16 && p.pack_width != 32 &&
+      p.pack_width != 64)
+    die("--pack-width must be 4, 8, 16, 32 or 64.");
+  if (p.pack_wg < 0 || p.pack_budget_gb < 0.0) die("--pack-wg and --pack-budget-gb must be non-negative.");
+  if (p.twist_grid < 0) die("--twist-grid must be non-negative.");
   const bool partial = p.dbg_nup >= 0 || p.dbg_ndn >= 0 || p.dbg_mx >= 0 || p.dbg_my >= 0 || !p.debug_csv.empty();
   const bool full = p.dbg_nup >= 0 && p.dbg_ndn >= 0 && p.dbg_mx >= 0 && p.dbg_my >= 0;
   if (partial && !full)

@@ -31,6 +31,13 @@ struct Options {
   int reduction = 1;  // 0 serial, 1 tree
   bool per_vector = false;
   std::string timing_json;
+  // packed small blocks: FTLM blocks of dimension <= pack_max_dim run
+  // batched in persistent launches (0 disables packing)
+  std::int64_t pack_max_dim = 131072;
+  int pack_width = 0, pack_wg = 0;  // 0: tuned on the first pack
+  double pack_budget_gb = 0.0;      // device bytes per pack (0: engine default)
+  // twist scan: phix, phiy in {0, 1/N, ..., (N-1)/N}, one CSV per twist
+  int twist_grid = 0;
 };
 
 // parse_args (main.cpp:257-368). Returns false when --help was printed.

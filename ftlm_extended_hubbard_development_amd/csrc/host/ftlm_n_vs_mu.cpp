@@ -4,11 +4,18 @@
 //
 // Same flags, stdout lines, CSV columns, checkpoint/.meta files and block
 // order (N_up, N_down, my, mx) as the reference. Per sector, the exact
-// blocks are diagonalised in one batched launch before the blocks are
-// visited in order, so records and accumulation keep the reference's order.
+// blocks are diagonalised in one batched launch. FTLM blocks up to
+// --pack-max-dim rows are added to a pack that runs as one persistent launch
+// once full (or at the end); larger blocks run on their own. Blocks are
+// finished (trace/debug output, checkpoint record, accumulation) strictly in
+// the reference's order from an in-order queue, so every output keeps the
+// reference's order whatever ran when. --twist-grid N visits every block for
+// all N*N twists, packing the twists of a block together (their start
+// vectors are shared: block seeds do not depend on the twist).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <deque>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -33,6 +40,36 @@ struct Ctx {
   ~Ctx() { ftlm_destroy(c); }
 };
 
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// One step of the ordered output: a block to finish or a sector line.
+struct Item {
+  bool is_line = false;
+  std::string line;
+  BlockKey key{};
+  int twist = 0;
+  std::int64_t dim = 0;
+  int particles = 0;
+  bool use_exact = false;
+  bool from_ckpt = false;
+  bool dbg_here = false;
+  Spectrum spec;
+  std::vector<double> exact;  // exact eigenvalues (exact, trace or debug blocks)
+  std::int64_t pack_idx = -1;
+  bool ready = false;
+};
+
+std::string twist_output(const std::string& out, double px, double py) {
+  std::ostringstream tag;
+  tag << ".phix" << px << "_phiy" << py;
+  const std::size_t slash = out.find_last_of('/');
+  const std::size_t dot = out.find_last_of('.');
+  if (dot == std::string::npos || (slash != std::string::npos && dot < slash)) return out + tag.str() + ".csv";
+  return out.substr(0, dot) + tag.str() + out.substr(dot);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -46,6 +83,18 @@ int main(int argc, char** argv) {
     const bool dbg = o.dbg_nup >= 0 && o.dbg_ndn >= 0 && o.dbg_mx >= 0 && o.dbg_my >= 0;
     if (!o.checkpoint.empty() && (trace || dbg))
       die("--checkpoint is not currently compatible with trace/debug block output.");
+    const bool scan = o.twist_grid > 0;
+    if (scan && (!o.checkpoint.empty() || trace || dbg))
+      die("--twist-grid is not compatible with --checkpoint or trace/debug block output.");
+    std::vector<std::pair<double, double>> twists;
+    if (scan) {
+      for (int iy = 0; iy < o.twist_grid; ++iy)
+        for (int ix = 0; ix < o.twist_grid; ++ix)
+          twists.push_back({(double)ix / o.twist_grid, (double)iy / o.twist_grid});
+    } else {
+      twists.push_back({o.phix, o.phiy});
+    }
+    const int ntw = (int)twists.size();
     check_or_write_metadata(o);
     ensure_header(o.checkpoint);
     std::uint64_t valid = 0;
@@ -63,15 +112,22 @@ int main(int argc, char** argv) {
     std::cout << " U=" << o.u << " V=" << o.v << " tx=" << o.tx << " ty=" << o.ty << " tp=" << o.tp
               << " phix=" << o.phix << " phiy=" << o.phiy << " threads=" << threads
               << " exact_block_threshold=" << o.threshold << "\n";
+    if (scan) std::cout << "twist grid " << o.twist_grid << "x" << o.twist_grid << " (" << ntw << " twists)\n";
     if (!o.checkpoint.empty())
       std::cout << "checkpoint=" << o.checkpoint << " loaded_blocks=" << ckpt.size() << "\n";
 
     Ctx ctx;
-    const ftlm_model model{o.lx, o.ly, o.tx, o.ty, o.tp, o.phix, o.phiy, o.u, o.v};
+    const ftlm_model model{o.lx, o.ly, o.tx, o.ty, o.tp, twists[0].first, twists[0].second, o.u, o.v};
     check(ftlm_create(&model, o.device, &ctx.c), "GPU engine unavailable");
+    int cur_tw = 0;
+    auto use_twist = [&](int t) {
+      if (t == cur_tw) return;
+      check(ftlm_set_twist(ctx.c, twists[t].first, twists[t].second), "twist");
+      cur_tw = t;
+    };
 
-    std::vector<Spectrum> stored;
-    Thermo single(mu.size());
+    std::vector<std::vector<Spectrum>> stored(ntw);
+    std::vector<Thermo> single(ntw, Thermo(mu.size()));
     std::ofstream tout;
     if (trace) {
       tout.open(o.trace_csv);
@@ -90,8 +146,162 @@ int main(int argc, char** argv) {
       tj << "[\n";
     }
     bool first_tj = true;
+    auto tj_rec = [&](const std::string& body) {
+      if (!tj) return;
+      tj << (first_tj ? "" : ",\n") << "{" << body << "}";
+      first_tj = false;
+    };
     const int L = sites;
     const std::size_t R = (std::size_t)o.samples;
+
+    // ---- finishing a block (the tail of the reference's block loop) ----
+    auto finish_item = [&](Item& it) {
+      if (it.is_line) {
+        std::cout << it.line;
+        return;
+      }
+      const int nup = it.key.nup, ndn = it.key.ndn, mx = it.key.mx, my = it.key.my;
+      const int particles = it.particles;
+      const std::int64_t dim = it.dim;
+      Spectrum& spec = it.spec;
+      const int t = it.twist;
+      if (it.from_ckpt) {
+        if (multi) stored[t].push_back(std::move(spec));
+        else
+          accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
+                     spec.particles, o.beta, mu, single[t], threads);
+        return;
+      }
+      const bool use_exact = it.use_exact;
+      const double* exact_eig = it.exact.empty() ? nullptr : it.exact.data();
+      const std::size_t exact_n = it.exact.size();
+      if (use_exact) {
+        spec.eig = it.exact;
+        spec.prefactor = 1.0;
+      }
+      const double sumw = use_exact ? 1.0 : std::accumulate(spec.w.begin(), spec.w.end(), 0.0);
+      if (trace) {
+        for (double m : mu) {
+          const BlockThermo used = use_exact ? raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m)
+                                             : raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
+                                                          (double)dim, particles, o.beta, m);
+          const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
+          const double ratio = ex.z > 0.0 ? used.z / ex.z : 0.0;
+          tout << nup << "," << ndn << "," << mx << "," << my << "," << particles << "," << dim << "," << m << ","
+               << (use_exact ? "exact" : "ftlm") << "," << used.z << "," << ex.z << "," << ratio << ","
+               << used.n_total << "," << ex.n_total << "," << (used.n_total - ex.n_total) << "," << used.n2_total
+               << "," << ex.n2_total << "," << (used.n2_total - ex.n2_total) << "," << used.min_shifted << ","
+               << ex.min_shifted << "," << sumw << "\n";
+        }
+      }
+      if (it.dbg_here) {
+        std::cout << "Selected block debug Nup=" << nup << " Ndown=" << ndn << " mx=" << mx << " my=" << my
+                  << " basis_dim=" << dim << " method=" << (use_exact ? "exact" : "ftlm")
+                  << " sum_sample_weights=" << sumw << "\n";
+        for (double m : mu) {
+          const BlockThermo used = use_exact ? raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m)
+                                             : raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
+                                                          (double)dim, particles, o.beta, m);
+          const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
+          const double ratio = ex.z > 0.0 ? used.z / ex.z : 0.0;
+          std::cout << "  debug mu=" << std::setw(12) << m << " z_used=" << std::setw(14) << used.z
+                    << " z_ed=" << std::setw(14) << ex.z << " ratio=" << std::setw(14) << ratio
+                    << " n_diff=" << std::setw(14) << (used.n_total - ex.n_total) << "\n";
+        }
+        if (!o.debug_csv.empty()) {
+          // write_selected_block_debug (main.cpp:1233-1267)
+          std::ofstream dout(o.debug_csv);
+          if (!dout) die("Failed to open selected block debug file: " + o.debug_csv);
+          dout << "nup,ndown,mx,my,basis_dim,mu,z_ftlm,z_ed,z_ratio,"
+               << "n_ftlm_total,n_ed_total,n_diff,n2_ftlm_total,n2_ed_total,n2_diff,"
+               << "min_ftlm,min_ed,sum_sample_weights\n";
+          dout << std::setprecision(15);
+          const bool have_ftlm = !use_exact;
+          const double sw = have_ftlm ? std::accumulate(spec.w.begin(), spec.w.end(), 0.0) : 0.0;
+          for (double m : mu) {
+            const BlockThermo f = have_ftlm ? raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
+                                                         (double)dim, particles, o.beta, m)
+                                            : BlockThermo{};
+            const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
+            const double ratio = ex.z > 0.0 ? f.z / ex.z : 0.0;
+            dout << nup << "," << ndn << "," << mx << "," << my << "," << dim << "," << m << "," << f.z << ","
+                 << ex.z << "," << ratio << "," << f.n_total << "," << ex.n_total << "," << (f.n_total - ex.n_total)
+                 << "," << f.n2_total << "," << ex.n2_total << "," << (f.n2_total - ex.n2_total) << ","
+                 << f.min_shifted << "," << ex.min_shifted << "," << sw << "\n";
+          }
+          wrote_dbg = true;
+        }
+      }
+      if (multi || !o.checkpoint.empty()) {
+        append_record(o.checkpoint, it.key, spec);
+        if (multi) {
+          stored[t].push_back(std::move(spec));
+        } else {
+          accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
+                     particles, o.beta, mu, single[t], threads);
+        }
+      } else {
+        accumulate(spec.eig.data(), use_exact ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor, particles,
+                   o.beta, mu, single[t], threads);
+      }
+    };
+
+    std::deque<Item> queue;
+
+    // ---- the pack of small FTLM blocks ----
+    const bool packing = o.pack_max_dim > 0 && !o.per_vector;
+    const int pack_method = o.reduction == FTLM_REDUCE_SERIAL ? 0 : 1;  // Jacobi (bit-exact) or QL Ritz
+    const std::uint64_t budget = (std::uint64_t)(o.pack_budget_gb * 1073741824.0);
+    std::vector<std::size_t> pending;  // push positions of the blocks in the current pack
+    std::int64_t pack_entries = 0;
+    bool tuned = o.pack_width > 0 && o.pack_wg > 0;
+    if (packing) check(ftlm_pack_begin(ctx.c, o.samples, o.steps, budget), "pack");
+    std::size_t popped = 0;  // items removed from the queue front so far
+    std::size_t pushed = 0;
+    auto push = [&](Item&& it) {
+      queue.push_back(std::move(it));
+      return pushed++;
+    };
+    auto drain = [&]() {
+      while (!queue.empty() && queue.front().ready) {
+        finish_item(queue.front());
+        queue.pop_front();
+        ++popped;
+      }
+    };
+    auto flush_pack = [&]() {
+      if (!packing || pack_entries == 0) return;
+      const auto t0 = std::chrono::steady_clock::now();
+      if (!tuned) {
+        int bw = 0, bg = 0, n = 0;
+        double bms = 0.0;
+        check(ftlm_pack_tune(ctx.c, 3, &bw, &bg, &bms, &n, nullptr, nullptr, nullptr), "pack tuning");
+        tuned = true;
+        tj_rec("\"method\":\"pack_tune\",\"width\":" + std::to_string(bw) + ",\"wg_per_cu\":" + std::to_string(bg));
+      }
+      check(ftlm_pack_run(ctx.c, o.pack_width, o.pack_wg, pack_method), "packed FTLM");
+      for (std::size_t pos : pending) {
+        Item& it = queue[pos - popped];
+        const std::size_t m = std::min<std::size_t>((std::size_t)o.steps, (std::size_t)it.dim);
+        it.spec.eig.resize(R * m);
+        it.spec.w.resize(R * m);
+        std::int64_t cnt = 0;
+        check(ftlm_pack_result(ctx.c, it.pack_idx, it.spec.eig.data(), it.spec.w.data(), &cnt), "packed FTLM");
+        it.spec.eig.resize((std::size_t)cnt);
+        it.spec.w.resize((std::size_t)cnt);
+        it.ready = true;
+      }
+      ftlm_stats st{};
+      ftlm_last_stats(ctx.c, &st);
+      std::ostringstream rec;
+      rec << "\"method\":\"pack\",\"blocks\":" << pack_entries << ",\"gpu_ms\":" << st.ms_total
+          << ",\"ms\":" << ms_since(t0) << ",\"bytes_apply\":" << st.bytes_apply;
+      tj_rec(rec.str());
+      pending.clear();
+      pack_entries = 0;
+      check(ftlm_pack_begin(ctx.c, o.samples, o.steps, budget), "pack");
+      drain();
+    };
 
     for (int nup = 0; nup <= L; ++nup) {
       for (int ndn = 0; ndn <= L; ++ndn) {
@@ -105,7 +315,7 @@ int main(int argc, char** argv) {
               << ", which exceeds --max-sector-dim=" << o.max_sector_dim;
           die(msg.str());
         }
-        // block dimensions and the exact batch of this sector
+        // block dimensions (twist independent) and the exact batch per twist
         std::vector<std::int64_t> dims(L);
         std::vector<int> exm, exy;
         std::vector<int> exidx(L, -1);
@@ -123,20 +333,20 @@ int main(int argc, char** argv) {
               exy.push_back(my);
             }
           }
-        std::vector<double> exeig;
+        std::vector<std::vector<double>> exeig(ntw);
         std::vector<std::int64_t> exoff(exm.size() + 1, 0);
         if (!exm.empty()) {
           std::int64_t tot = 0;
           for (std::size_t i = 0; i < exm.size(); ++i) tot += dims[exy[i] * o.lx + exm[i]];
-          exeig.resize((std::size_t)tot);
-          const auto t0 = std::chrono::steady_clock::now();
-          check(ftlm_exact_blocks(ctx.c, (int)exm.size(), exm.data(), exy.data(), exeig.data(), exoff.data()),
-                "exact diagonalization");
-          if (tj) {
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            tj << (first_tj ? "" : ",\n") << "{\"nup\":" << nup << ",\"ndn\":" << ndn << ",\"method\":\"exact_batch\","
-               << "\"blocks\":" << exm.size() << ",\"ms\":" << ms << "}";
-            first_tj = false;
+          for (int t = 0; t < ntw; ++t) {
+            use_twist(t);
+            exeig[t].resize((std::size_t)tot);
+            const auto t0 = std::chrono::steady_clock::now();
+            check(ftlm_exact_blocks(ctx.c, (int)exm.size(), exm.data(), exy.data(), exeig[t].data(), exoff.data()),
+                  "exact diagonalization");
+            tj_rec("\"nup\":" + std::to_string(nup) + ",\"ndn\":" + std::to_string(ndn) +
+                   ",\"method\":\"exact_batch\",\"blocks\":" + std::to_string(exm.size()) +
+                   ",\"ms\":" + std::to_string(ms_since(t0)));
           }
         }
         std::size_t active = 0;
@@ -146,155 +356,113 @@ int main(int argc, char** argv) {
             const std::int64_t dim = dims[k];
             active += (std::size_t)dim;
             if (dim == 0) continue;
-            const int particles = nup + ndn;
-            const BlockKey key{nup, ndn, mx, my};
-            const auto hit = ckpt.find(key);
-            if (hit != ckpt.end()) {
-              const Spectrum& s = hit->second;
-              if (s.dim != (std::uint64_t)dim || s.particles != particles)
-                die("Checkpoint block metadata does not match current run.");
-              if (multi) stored.push_back(s);
-              else
-                accumulate(s.eig.data(), s.w.empty() ? nullptr : s.w.data(), s.eig.size(), s.prefactor, s.particles,
-                           o.beta, mu, single, threads);
-              continue;
-            }
-            const bool use_exact = (std::size_t)dim <= o.threshold;
-            Spectrum spec;
-            spec.particles = particles;
-            spec.dim = (std::uint64_t)dim;
-            spec.exact = use_exact;
-            if (!use_exact) {
-              const auto t0 = std::chrono::steady_clock::now();
+            for (int t = 0; t < ntw; ++t) {
+              Item it;
+              it.key = BlockKey{nup, ndn, mx, my};
+              it.twist = t;
+              it.dim = dim;
+              it.particles = nup + ndn;
+              const auto hit = ckpt.find(it.key);
+              if (hit != ckpt.end()) {
+                const Spectrum& s = hit->second;
+                if (s.dim != (std::uint64_t)dim || s.particles != it.particles)
+                  die("Checkpoint block metadata does not match current run.");
+                it.spec = s;
+                it.from_ckpt = true;
+                it.ready = true;
+                push(std::move(it));
+                continue;
+              }
+              it.use_exact = (std::size_t)dim <= o.threshold;
+              it.dbg_here = dbg && nup == o.dbg_nup && ndn == o.dbg_ndn && mx == o.dbg_mx && my == o.dbg_my;
+              it.spec.particles = it.particles;
+              it.spec.dim = (std::uint64_t)dim;
+              it.spec.exact = it.use_exact;
+              if (exidx[k] >= 0)
+                it.exact.assign(exeig[t].begin() + exoff[exidx[k]], exeig[t].begin() + exoff[exidx[k] + 1]);
+              if (it.use_exact) {
+                it.ready = true;
+                push(std::move(it));
+                continue;
+              }
+              it.spec.prefactor = (double)dim;
+              use_twist(t);
               check(ftlm_block(ctx.c, mx, my, nullptr), "block build");
-              const std::size_t m = std::min<std::size_t>((std::size_t)o.steps, (std::size_t)dim);
-              spec.eig.resize(R * m);
-              spec.w.resize(R * m);
-              std::int64_t cnt = 0;
-              check(ftlm_ftlm_block(ctx.c, o.samples, o.steps, ftlm_block_seed(o.seed, nup, ndn, mx, my), o.reduction,
-                                    o.per_vector ? 1 : 0, spec.eig.data(), spec.w.data(), &cnt),
-                    "FTLM block");
-              spec.eig.resize((std::size_t)cnt);
-              spec.w.resize((std::size_t)cnt);
-              spec.prefactor = (double)dim;
-              if (tj) {
-                const double ms =
-                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                tj << (first_tj ? "" : ",\n") << "{\"nup\":" << nup << ",\"ndn\":" << ndn << ",\"mx\":" << mx
-                   << ",\"my\":" << my << ",\"dim\":" << dim << ",\"method\":\"ftlm\",\"ms\":" << ms << "}";
-                first_tj = false;
-              }
-            }
-            const double* exact_eig = nullptr;
-            std::size_t exact_n = 0;
-            if (exidx[k] >= 0) {
-              exact_eig = exeig.data() + exoff[exidx[k]];
-              exact_n = (std::size_t)(exoff[exidx[k] + 1] - exoff[exidx[k]]);
-            }
-            if (use_exact) {
-              spec.eig.assign(exact_eig, exact_eig + exact_n);
-              spec.prefactor = 1.0;
-            }
-            const double sumw = use_exact ? 1.0 : std::accumulate(spec.w.begin(), spec.w.end(), 0.0);
-            if (trace) {
-              for (double m : mu) {
-                const BlockThermo used = use_exact
-                                             ? raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m)
-                                             : raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
-                                                          (double)dim, particles, o.beta, m);
-                const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
-                const double ratio = ex.z > 0.0 ? used.z / ex.z : 0.0;
-                tout << nup << "," << ndn << "," << mx << "," << my << "," << particles << "," << dim << "," << m
-                     << "," << (use_exact ? "exact" : "ftlm") << "," << used.z << "," << ex.z << "," << ratio << ","
-                     << used.n_total << "," << ex.n_total << "," << (used.n_total - ex.n_total) << ","
-                     << used.n2_total << "," << ex.n2_total << "," << (used.n2_total - ex.n2_total) << ","
-                     << used.min_shifted << "," << ex.min_shifted << "," << sumw << "\n";
-              }
-            }
-            if (dbg && nup == o.dbg_nup && ndn == o.dbg_ndn && mx == o.dbg_mx && my == o.dbg_my) {
-              std::cout << "Selected block debug Nup=" << nup << " Ndown=" << ndn << " mx=" << mx << " my=" << my
-                        << " basis_dim=" << dim << " method=" << (use_exact ? "exact" : "ftlm")
-                        << " sum_sample_weights=" << sumw << "\n";
-              for (double m : mu) {
-                const BlockThermo used = use_exact
-                                             ? raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m)
-                                             : raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
-                                                          (double)dim, particles, o.beta, m);
-                const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
-                const double ratio = ex.z > 0.0 ? used.z / ex.z : 0.0;
-                std::cout << "  debug mu=" << std::setw(12) << m << " z_used=" << std::setw(14) << used.z
-                          << " z_ed=" << std::setw(14) << ex.z << " ratio=" << std::setw(14) << ratio
-                          << " n_diff=" << std::setw(14) << (used.n_total - ex.n_total) << "\n";
-              }
-              if (!o.debug_csv.empty()) {
-                // write_selected_block_debug (main.cpp:1233-1267)
-                std::ofstream dout(o.debug_csv);
-                if (!dout) die("Failed to open selected block debug file: " + o.debug_csv);
-                dout << "nup,ndown,mx,my,basis_dim,mu,z_ftlm,z_ed,z_ratio,"
-                     << "n_ftlm_total,n_ed_total,n_diff,n2_ftlm_total,n2_ed_total,n2_diff,"
-                     << "min_ftlm,min_ed,sum_sample_weights\n";
-                dout << std::setprecision(15);
-                const bool have_ftlm = !use_exact;
-                const double sw = have_ftlm ? std::accumulate(spec.w.begin(), spec.w.end(), 0.0) : 0.0;
-                for (double m : mu) {
-                  const BlockThermo f = have_ftlm ? raw_thermo(spec.eig.data(), spec.w.data(), spec.eig.size(),
-                                                               (double)dim, particles, o.beta, m)
-                                                  : BlockThermo{};
-                  const BlockThermo ex = raw_thermo(exact_eig, nullptr, exact_n, 1.0, particles, o.beta, m);
-                  const double ratio = ex.z > 0.0 ? f.z / ex.z : 0.0;
-                  dout << nup << "," << ndn << "," << mx << "," << my << "," << dim << "," << m << "," << f.z << ","
-                       << ex.z << "," << ratio << "," << f.n_total << "," << ex.n_total << ","
-                       << (f.n_total - ex.n_total) << "," << f.n2_total << "," << ex.n2_total << ","
-                       << (f.n2_total - ex.n2_total) << "," << f.min_shifted << "," << ex.min_shifted << "," << sw
-                       << "\n";
+              const std::uint64_t bseed = ftlm_block_seed(o.seed, nup, ndn, mx, my);
+              if (packing && dim <= o.pack_max_dim) {
+                int rc = ftlm_pack_add(ctx.c, bseed, &it.pack_idx);
+                if (rc == FTLM_PACK_FULL) {
+                  flush_pack();
+                  rc = ftlm_pack_add(ctx.c, bseed, &it.pack_idx);
                 }
-                wrote_dbg = true;
+                check(rc == FTLM_PACK_FULL ? FTLM_ERR_LIMIT : rc, "pack");
+                ++pack_entries;
+                pending.push_back(push(std::move(it)));
+                continue;
               }
+              const auto t0 = std::chrono::steady_clock::now();
+              const std::size_t m = std::min<std::size_t>((std::size_t)o.steps, (std::size_t)dim);
+              it.spec.eig.resize(R * m);
+              it.spec.w.resize(R * m);
+              std::int64_t cnt = 0;
+              check(ftlm_ftlm_block(ctx.c, o.samples, o.steps, bseed, o.reduction, o.per_vector ? 1 : 0,
+                                    it.spec.eig.data(), it.spec.w.data(), &cnt),
+                    "FTLM block");
+              it.spec.eig.resize((std::size_t)cnt);
+              it.spec.w.resize((std::size_t)cnt);
+              it.ready = true;
+              tj_rec("\"nup\":" + std::to_string(nup) + ",\"ndn\":" + std::to_string(ndn) + ",\"mx\":" +
+                     std::to_string(mx) + ",\"my\":" + std::to_string(my) + ",\"dim\":" + std::to_string(dim) +
+                     ",\"method\":\"ftlm\",\"ms\":" + std::to_string(ms_since(t0)));
+              push(std::move(it));
             }
-            if (multi || !o.checkpoint.empty()) {
-              append_record(o.checkpoint, key, spec);
-              if (multi) {
-                stored.push_back(std::move(spec));
-              } else {
-                accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
-                           particles, o.beta, mu, single, threads);
-              }
-            } else {
-              accumulate(spec.eig.data(), use_exact ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
-                         particles, o.beta, mu, single, threads);
-            }
+            drain();
           }
         }
-        std::cout << "  sector Nup=" << nup << " Ndown=" << ndn << " full_dim=" << full << " parents=" << parents
-                  << " active_k_dim_sum=" << active << "\n";
+        std::ostringstream line;
+        line << "  sector Nup=" << nup << " Ndown=" << ndn << " full_dim=" << full << " parents=" << parents
+             << " active_k_dim_sum=" << active << "\n";
+        Item li;
+        li.is_line = true;
+        li.line = line.str();
+        li.ready = true;
+        push(std::move(li));
+        drain();
       }
     }
+    flush_pack();
+    drain();
+    if (!queue.empty()) die("internal error: unfinished blocks");
     if (tj) tj << "\n]\n";
     if (dbg && !o.debug_csv.empty() && !wrote_dbg)
       die("Selected block debug target was not found among active FTLM blocks.");
 
-    if (multi) {
-      std::vector<Results> all;
-      for (double beta : o.betas) {
-        Thermo t(mu.size());
-        for (const Spectrum& s : stored)
-          accumulate(s.eig.data(), s.w.empty() ? nullptr : s.w.data(), s.eig.size(), s.prefactor, s.particles, beta,
-                     mu, t, threads);
-        Results r = finalize(t, beta, sites);
+    for (int t = 0; t < ntw; ++t) {
+      const std::string out = scan ? twist_output(o.output, twists[t].first, twists[t].second) : o.output;
+      if (scan) std::cout << "twist phix=" << twists[t].first << " phiy=" << twists[t].second << "\n";
+      if (multi) {
+        std::vector<Results> all;
+        for (double beta : o.betas) {
+          Thermo th(mu.size());
+          for (const Spectrum& s : stored[t])
+            accumulate(s.eig.data(), s.w.empty() ? nullptr : s.w.data(), s.eig.size(), s.prefactor, s.particles,
+                       beta, mu, th, threads);
+          Results r = finalize(th, beta, sites);
+          for (std::size_t i = 0; i < mu.size(); ++i)
+            std::cout << "beta=" << std::setw(12) << beta << "  mu=" << std::setw(12) << mu[i]
+                      << "  n=" << std::setw(12) << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
+          all.push_back(std::move(r));
+        }
+        write_multi_beta(out, o.betas, mu, all);
+      } else {
+        const Results r = finalize(single[t], o.beta, sites);
         for (std::size_t i = 0; i < mu.size(); ++i)
-          std::cout << "beta=" << std::setw(12) << beta << "  mu=" << std::setw(12) << mu[i] << "  n=" << std::setw(12)
-                    << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
-        all.push_back(std::move(r));
+          std::cout << "beta=" << std::setw(12) << o.beta << "  mu=" << std::setw(12) << mu[i]
+                    << "  n=" << std::setw(12) << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
+        write_results(out, mu, r);
       }
-      write_multi_beta(o.output, o.betas, mu, all);
-    } else {
-      const Results r = finalize(single, o.beta, sites);
-      for (std::size_t i = 0; i < mu.size(); ++i)
-        std::cout << "beta=" << std::setw(12) << o.beta << "  mu=" << std::setw(12) << mu[i] << "  n=" << std::setw(12)
-                  << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
-      write_results(o.output, mu, r);
+      std::cout << "Wrote " << out << "\n";
     }
-    std::cout << "Wrote " << o.output << "\n";
     return 0;
   } catch (const std::exception& ex) {
     std::cerr << "Error: " << ex.what() << "\n";

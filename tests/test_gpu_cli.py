@@ -168,3 +168,135 @@ def test_ed_4x2_bounds_ftlm(tmp_path):
     ftlm = load(os.path.join(GOLD, "csv", "cfg2_4x2.csv"))
     ed = load(out)
     assert np.max(np.abs(ftlm["n"] - ed["n"])) < 1e-2
+
+
+@pytest.mark.parametrize("pack", [["--pack-max-dim", "0"], ["--pack-max-dim", "100"],
+                                  ["--pack-width", "8", "--pack-wg", "1"], ["--pack-budget-gb", "0.002"]],
+                         ids=["unpacked", "mixed", "w8", "many-packs"])
+@pytest.mark.parametrize("name", ["cfg2_4x2", "4x2_multibeta_r8_s60"])
+def test_cli_pack_configurations(tmp_path, name, pack):
+    """Packed, unpacked and mixed block execution give the reference's
+    output: byte for byte in serial mode when every block is FTLM, within the
+    stated tolerances otherwise."""
+    if name not in CASES:
+        pytest.skip("no golden")
+    out = str(tmp_path / "out.csv")
+    gold = os.path.join(GOLD, "csv", name + ".csv")
+    if CASES[name]["exact_blocks_from"] == "none":
+        run(EXE, CASES[name]["args"] + ["--output", out, "--reduction", "serial"] + pack, str(tmp_path))
+        assert open(out, "rb").read() == open(gold, "rb").read()
+    out2 = str(tmp_path / "out2.csv")
+    run(EXE, CASES[name]["args"] + ["--output", out2] + pack, str(tmp_path))
+    compare(out2, gold)
+
+
+TW_ARGS = ["--lx", "3", "--ly", "3", "--tp", "-0.25", "--u", "6", "--v", "1", "--beta", "4", "--samples", "4",
+           "--lanczos-steps", "30", "--mu-min", "-2", "--mu-max", "8", "--mu-count", "11",
+           "--exact-block-threshold", "0"]
+
+
+def test_cli_twist_grid_equals_separate_runs(tmp_path):
+    """--twist-grid 2 runs the four twists {0, 1/2}^2 in one process with
+    their blocks packed together; each per-twist CSV equals the reference
+    run at that twist (the oracle's CLI), byte for byte in serial mode."""
+    orc = oracle_lib.oracle()
+    out = str(tmp_path / "tw.csv")
+    stdout = run(EXE, TW_ARGS + ["--twist-grid", "2", "--reduction", "serial", "--output", out], str(tmp_path))
+    assert "twist grid 2x2 (4 twists)" in stdout
+    for px, py in [(0, 0), (0.5, 0), (0, 0.5), (0.5, 0.5)]:
+        got = str(tmp_path / f"tw.phix{px:g}_phiy{py:g}.csv")
+        assert os.path.exists(got), got
+        ref = str(tmp_path / f"ref_{px}_{py}.csv")
+        rc = orc.main(TW_ARGS + ["--phix", str(px), "--phiy", str(py), "--output", ref])
+        assert rc == 0
+        assert open(got, "rb").read() == open(ref, "rb").read(), (px, py)
+    # tree reductions (QL Ritz in the packs): within the stated tolerances
+    out2 = str(tmp_path / "tt.csv")
+    run(EXE, TW_ARGS + ["--twist-grid", "2", "--output", out2], str(tmp_path))
+    for px, py in [(0, 0), (0.5, 0), (0, 0.5), (0.5, 0.5)]:
+        compare(str(tmp_path / f"tt.phix{px:g}_phiy{py:g}.csv"), str(tmp_path / f"ref_{px}_{py}.csv"))
+
+
+def _raw_thermo(e, w, pref, N, beta, mu):
+    sh = e - mu * N
+    m = sh.min()
+    wt = pref * w * np.exp(-beta * (sh - m)) if w is not None else np.exp(-beta * (sh - m))
+    return wt.sum(), N * wt.sum(), N * N * wt.sum(), m
+
+
+TR_ARGS = CASES["cfg1_2x2"]["args"]  # 2x2, beta 1.5, 5 samples, 40 steps, 33 mu points
+TR_MODEL = dict(lx=2, ly=2, tp=0.2, u=4.0, v=1.0)
+
+
+@pytest.mark.parametrize("threshold", ["0", "256"])
+def test_cli_trace_partition_csv(tmp_path, threshold):
+    """--trace-partition-csv: per block and mu, the used (FTLM or exact) and
+    the exact partition sums (main.cpp:1505-1663), against the oracle's
+    spectra and numpy's eigenvalues of the reference's dense blocks."""
+    orc = oracle_lib.oracle()
+    tr = str(tmp_path / "trace.csv")
+    out = str(tmp_path / "out.csv")
+    run(EXE, TR_ARGS + ["--exact-block-threshold", threshold, "--reduction", "serial", "--trace-partition-csv", tr,
+                        "--output", out], str(tmp_path))
+    rows = list(__import__("csv").DictReader(open(tr)))
+    m = oracle_lib.model(**TR_MODEL)
+    mu = orc.linspace(-2.0, 6.0, 33)
+    k = 0
+    for nup in range(5):
+        for ndn in range(5):
+            for my in range(2):
+                for mx in range(2):
+                    blk = orc.block(m, nup, ndn, mx, my)
+                    if blk.dim == 0:
+                        continue
+                    ed = np.linalg.eigvalsh(blk.dense())
+                    ftlm = blk.dim > int(threshold)
+                    if ftlm:
+                        e, w = orc.ftlm_block(blk, 5, 40, orc.block_seed(12345, nup, ndn, mx, my))
+                    for mv in mu:
+                        r = rows[k]
+                        k += 1
+                        assert (int(r["nup"]), int(r["ndown"]), int(r["mx"]), int(r["my"])) == (nup, ndn, mx, my)
+                        assert int(r["basis_dim"]) == blk.dim and r["method"] == ("ftlm" if ftlm else "exact")
+                        N = nup + ndn
+                        zu = _raw_thermo(e, w, blk.dim, N, 1.5, mv) if ftlm else _raw_thermo(ed, None, 1, N, 1.5, mv)
+                        ze = _raw_thermo(ed, None, 1, N, 1.5, mv)
+                        assert abs(float(r["z_used"]) - zu[0]) <= 1e-10 * abs(zu[0])
+                        assert abs(float(r["z_ed"]) - ze[0]) <= 1e-10 * abs(ze[0])
+                        assert abs(float(r["n_used_total"]) - zu[1]) <= 1e-10 * abs(zu[1]) + 1e-300
+                        assert abs(float(r["min_ftlm"]) - zu[3]) <= 1e-12 * (1 + abs(zu[3]))
+                        assert abs(float(r["min_ed"]) - ze[3]) <= 1e-12 * (1 + abs(ze[3]))
+                        sw = w.sum() if ftlm else 1.0
+                        assert abs(float(r["sum_sample_weights"]) - sw) <= 1e-13
+    assert k == len(rows)
+    compare(out, os.path.join(GOLD, "csv", "cfg1_2x2_ftlm_only.csv" if threshold == "0" else "cfg1_2x2.csv"))
+
+
+def test_cli_debug_block(tmp_path):
+    """--debug-block-*: the selected FTLM block's comparison with its exact
+    spectrum (stdout lines and CSV of main.cpp:1233-1267, 1627-1663)."""
+    orc = oracle_lib.oracle()
+    dbg = str(tmp_path / "dbg.csv")
+    out = str(tmp_path / "out.csv")
+    stdout = run(EXE, TR_ARGS + ["--exact-block-threshold", "0", "--reduction", "serial", "--debug-block-nup", "2",
+                                 "--debug-block-ndown", "1", "--debug-block-mx", "1", "--debug-block-my", "0",
+                                 "--debug-block-csv", dbg, "--output", out], str(tmp_path))
+    blk = orc.block(oracle_lib.model(**TR_MODEL), 2, 1, 1, 0)
+    assert f"Selected block debug Nup=2 Ndown=1 mx=1 my=0 basis_dim={blk.dim} method=ftlm" in stdout
+    assert stdout.count("  debug mu=") == 33
+    rows = list(__import__("csv").DictReader(open(dbg)))
+    assert len(rows) == 33
+    e, w = orc.ftlm_block(blk, 5, 40, orc.block_seed(12345, 2, 1, 1, 0))
+    ed = np.linalg.eigvalsh(blk.dense())
+    for r, mv in zip(rows, orc.linspace(-2.0, 6.0, 33)):
+        zf = _raw_thermo(e, w, blk.dim, 3, 1.5, mv)
+        ze = _raw_thermo(ed, None, 1, 3, 1.5, mv)
+        assert abs(float(r["z_ftlm"]) - zf[0]) <= 1e-10 * zf[0] and abs(float(r["z_ed"]) - ze[0]) <= 1e-10 * ze[0]
+        assert abs(float(r["n_diff"]) - (zf[1] - ze[1])) <= 1e-9 * max(zf[1], ze[1])
+    compare(out, os.path.join(GOLD, "csv", "cfg1_2x2_ftlm_only.csv"))
+    # a debug target that is not a block fails like the reference
+    p = subprocess.run([EXE] + TR_ARGS + ["--debug-block-nup", "9", "--debug-block-ndown", "1", "--debug-block-mx",
+                                          "0", "--debug-block-my", "0", "--debug-block-csv", dbg,
+                                          "--output", out], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=300)
+    assert p.returncode == 1 and "Selected block debug target was not found" in p.stderr
