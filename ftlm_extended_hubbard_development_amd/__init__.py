@@ -28,12 +28,18 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lp = C.POINTER(C.c_int64)
 _up = C.POINTER(C.c_uint64)
+_fp = C.POINTER(C.c_float)
 
 
 class FtlmModel(C.Structure):
     _fields_ = [("lx", C.c_int32), ("ly", C.c_int32), ("tx", C.c_double), ("ty", C.c_double),
                 ("tp", C.c_double), ("phix", C.c_double), ("phiy", C.c_double),
                 ("u", C.c_double), ("v", C.c_double)]
+
+
+class FtlmMlpConfig(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("y_mean", C.c_double * 2), ("y_scale", C.c_double * 2)]
 
 
 class FtlmStats(C.Structure):
@@ -71,6 +77,21 @@ SIGNATURES = {
     "ftlm_pack_result": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _lp]),
     "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
     "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
+    # learned surrogate
+    "ftlm_mlp_param_count": (C.c_int64, []),
+    "ftlm_mlp_layout": (C.c_int, [_lp, _lp, _ip, _ip]),
+    "ftlm_mlp_encode": (C.c_int, [C.c_int64, _dp, _fp]),
+    "ftlm_mlp_create": (C.c_int, [C.POINTER(FtlmMlpConfig), C.c_int, C.POINTER(C.c_void_p)]),
+    "ftlm_mlp_destroy": (None, [C.c_void_p]),
+    "ftlm_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "ftlm_mlp_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "ftlm_mlp_set_lr": (C.c_int, [C.c_void_p, C.c_float]),
+    "ftlm_mlp_set_data": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp]),
+    "ftlm_mlp_train": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, _fp]),
+    "ftlm_mlp_gradients": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _fp, _fp]),
+    "ftlm_mlp_step": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _fp]),
+    "ftlm_mlp_predict": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp]),
+    "ftlm_mlp_gemm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]),
 }
 
 PACK_FULL = 1

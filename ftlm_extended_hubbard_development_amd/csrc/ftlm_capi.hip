@@ -1189,10 +1189,13 @@ bool pack_add(ftlm_ctx* c, uint64_t bseed, int64_t* index) {
 
 }  // namespace
 
+void ftlm::set_last_error(const char* msg) { g_err = msg; }
+
 // ============================================================================
 extern "C" {
 
 const char* ftlm_last_error(void) { return g_err.c_str(); }
+
 
 uint64_t ftlm_block_seed(uint64_t seed, int nup, int ndn, int mx, int my) {
   uint64_t s = mix64(seed);

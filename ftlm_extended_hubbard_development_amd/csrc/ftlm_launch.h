@@ -9,6 +9,9 @@
 
 namespace ftlm {
 
+// Message returned by ftlm_last_error() (shared by every C ABI source file).
+void set_last_error(const char* msg);
+
 // ---- basis (ftlm_basis.hip) ----
 // Per up configuration: orbit minimum, shifts reaching it, their sign.
 void launch_up_table(const LatticeConst* g, const uint64_t* binom, int nup, int64_t n_upc, uint8_t* is_canon,

@@ -1,0 +1,868 @@
+// ftlm_mlp.hip -- learned surrogate of the FTLM thermodynamics (declared in
+// include/ftlm.h, section "learned surrogate").
+//
+// An MLP maps the model and ensemble parameters (U, V, t', phix, phiy, beta,
+// mu) to the per-site density n and the compressibility kappa that the
+// engine's FTLM sweeps produce:
+//
+//   12 encoded features (zero padded to 32)
+//     -> 4 hidden layers of 256 units, GELU (erf form)
+//     -> 2 outputs (padded to 32), in units of a per-output normalisation.
+//
+// Weights are bf16 copies of fp32 master parameters; every product (forward,
+// both backward products, weight gradients) is a bf16 x bf16 MFMA with fp32
+// accumulation (v_mfma_f32_16x16x32_bf16 on gfx950); biases, activations'
+// pre-images, the loss and the Adam state stay fp32. One training step is a
+// fixed launch sequence captured once into a HIP graph and replayed; batch
+// indices and the step counter live in device memory so that every replay
+// sees its own batch.
+//
+// Layouts. A GEMM here is always C[M x N] = A[M x K] * B[N x K]^T with both
+// operands K-contiguous bf16 (so each lane of a wavefront loads 8
+// consecutive K values, 16 bytes, for either operand). Every product of the
+// network takes this form when producers also write transposed copies:
+//   forward      Z_j  = H_j  W_j^T          A = H_j  [B x K_j],  B = W_j   [N_j x K_j]
+//   data grad    dH_j = dZ_j W_j            A = dZ_j [B x N_j],  B = W_j^T [K_j x N_j]
+//   weight grad  dW_j = dZ_j^T H_j          A = dZ_j^T [N_j x B], B = H_j^T [K_j x B]
+// so activations and their gradients are stored twice ([B x n] and [n x B])
+// and the optimiser writes each bf16 weight matrix in both orientations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ftlm.h"
+#include "ftlm_launch.h"
+
+namespace {
+
+// ---------------------------------------------------------------- shapes
+constexpr int kFeat = FTLM_MLP_FEATURES;  // 32: padded input width (one MFMA K step)
+constexpr int kHid = 256;
+constexpr int kNHid = 4;
+constexpr int kOutPad = 32;                // padded output width (K of the first backward product)
+constexpr int kLayers = kNHid + 1;
+
+// layer l: out x in
+constexpr int layer_out(int l) { return l < kNHid ? kHid : kOutPad; }
+constexpr int layer_in(int l) { return l == 0 ? kFeat : kHid; }
+// flat fp32 parameters: W_0, b_0, W_1, b_1, ..., W_4, b_4 (W row-major out x in)
+constexpr int64_t w_off(int l) {
+  int64_t o = 0;
+  for (int i = 0; i < l; ++i) o += (int64_t)layer_out(i) * layer_in(i) + layer_out(i);
+  return o;
+}
+constexpr int64_t b_off(int l) { return w_off(l) + (int64_t)layer_out(l) * layer_in(l); }
+constexpr int64_t kParams = w_off(kLayers);
+
+// --------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MLPCHECK(expr)                                                                               \
+  do {                                                                                               \
+    const hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                            \
+      throw Error(e_ == hipErrorOutOfMemory ? FTLM_ERR_MEMORY : FTLM_ERR_DEVICE,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                                \
+  } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return FTLM_OK;
+  } catch (const Error& e) {
+    ftlm::set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ftlm::set_last_error("host allocation failed");
+    return FTLM_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    ftlm::set_last_error(e.what());
+    return FTLM_ERR_ARG;
+  }
+}
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* get(size_t count) {
+    const size_t n = std::max<size_t>(count * sizeof(T), 256);
+    if (n > bytes) {
+      if (p) MLPCHECK(hipFree(p));
+      p = nullptr;
+      bytes = 0;
+      MLPCHECK(hipMalloc(&p, n));
+      bytes = n;
+    }
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ------------------------------------------------------------ device math
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+
+// fp32 -> bf16, round to nearest even (NaN kept quiet)
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// d/dz [z Phi(z)] = Phi(z) + z phi(z)
+__device__ inline float dgelu(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__device__ inline bf16x8 ld8(const uint16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// ------------------------------------------------------------------ GEMM
+enum { EPI_F32 = 0, EPI_GELU_FWD = 1, EPI_OUT = 2, EPI_GELU_BWD = 3 };
+
+struct Epi {
+  const float* bias;  // [N] (GELU_FWD, OUT)
+  float* zf;          // fp32 [M x ldz]: output (F32, OUT, GELU_FWD pre-activation) or input (GELU_BWD)
+  int ldz;
+  uint16_t* hb;       // bf16 [M x ldh] (GELU_FWD, GELU_BWD)
+  int ldh;
+  uint16_t* ht;       // bf16 transposed [N x ldt] (optional)
+  int ldt;
+};
+
+// One workgroup = 4 wavefronts = a 64-row x 64-column tile of C; wavefront w
+// owns rows 16w..16w+15 and the tile's four 16 x 16 column blocks, stepping
+// K by 32 with one v_mfma_f32_16x16x32_bf16 per column block. Operand
+// fragments (lane l: row l % 16, K values 8 (l / 16) .. +7) are loaded
+// straight from global memory (the whole network is L2-resident); the
+// accumulator fragment of lane l holds C rows 4 (l / 16) .. +3, column l % 16.
+// Requires M % 16 == 0, N % 16 == 0, K % 32 == 0, lda/ldb % 8 == 0.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B,
+                                              int ldb, int M, int N, int K, Epi e) {
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int m0 = blockIdx.y * 64 + wave * 16;
+  const int n0 = blockIdx.x * 64;
+  if (m0 >= M) return;  // whole wavefront
+  const int r = lane & 15;
+  const int kq = (lane >> 4) * 8;
+  int nt = (N - n0) / 16;
+  if (nt > 4) nt = 4;
+  f32x4 acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* pa = A + (size_t)(m0 + r) * lda + kq;
+  const uint16_t* pb = B + (size_t)(n0 + r) * ldb + kq;
+  const size_t bstep = (size_t)16 * ldb;
+  if (nt == 4) {
+    for (int k = 0; k < K; k += 32) {
+      const bf16x8 a = ld8(pa + k);
+      const bf16x8 b0 = ld8(pb + k);
+      const bf16x8 b1 = ld8(pb + bstep + k);
+      const bf16x8 b2 = ld8(pb + 2 * bstep + k);
+      const bf16x8 b3 = ld8(pb + 3 * bstep + k);
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b3, acc[3], 0, 0, 0);
+    }
+  } else {
+    for (int k = 0; k < K; k += 32) {
+      const bf16x8 a = ld8(pa + k);
+      for (int t = 0; t < nt; ++t) {
+        const bf16x8 b = ld8(pb + t * bstep + k);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int mr = m0 + (lane >> 4) * 4;
+  for (int t = 0; t < nt; ++t) {
+    const int n = n0 + 16 * t + r;
+    for (int i = 0; i < 4; ++i) {
+      const int m = mr + i;
+      const float v = acc[t][i];
+      if (EPI == EPI_F32) {
+        e.zf[(size_t)m * e.ldz + n] = v;
+      } else if (EPI == EPI_OUT) {
+        e.zf[(size_t)m * e.ldz + n] = v + e.bias[n];
+      } else if (EPI == EPI_GELU_FWD) {
+        const float z = v + e.bias[n];
+        if (e.zf) e.zf[(size_t)m * e.ldz + n] = z;
+        const uint16_t h = f2bf(gelu(z));
+        e.hb[(size_t)m * e.ldh + n] = h;
+        if (e.ht) e.ht[(size_t)n * e.ldt + m] = h;
+      } else {  // EPI_GELU_BWD
+        const float d = v * dgelu(e.zf[(size_t)m * e.ldz + n]);
+        const uint16_t h = f2bf(d);
+        e.hb[(size_t)m * e.ldh + n] = h;
+        if (e.ht) e.ht[(size_t)n * e.ldt + m] = h;
+      }
+    }
+  }
+}
+
+template <int EPI>
+void gemm(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, int K, const Epi& e,
+          hipStream_t s) {
+  if (M % 16 || N % 16 || K % 32 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || K <= 0)
+    throw Error(FTLM_ERR_ARG, "MLP GEMM shape not supported");
+  dim3 grid((N + 63) / 64, (M + 63) / 64);
+  hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, A, lda, B, ldb, M, N, K, e);
+  MLPCHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------- other kernels
+// fp32 rows [n x kFeat] -> bf16 rows [rows x kFeat] (+ transposed [kFeat x
+// rows]); rows n..rows-1 are zero.
+__global__ void k_feat_bf16(const float* __restrict__ f, int64_t n, int64_t rows, uint16_t* __restrict__ X,
+                            uint16_t* __restrict__ Xt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * kFeat) return;
+  const int64_t b = i / kFeat;
+  const int c = (int)(i % kFeat);
+  const uint16_t h = b < n ? f2bf(f[i]) : (uint16_t)0;
+  X[i] = h;
+  if (Xt) Xt[(size_t)c * rows + b] = h;
+}
+
+// Training batch `iter` of the index list: feature rows and targets.
+__global__ void k_gather(const uint16_t* __restrict__ feat, const float* __restrict__ tgt,
+                         const int32_t* __restrict__ idx, const int32_t* __restrict__ iter, int B,
+                         uint16_t* __restrict__ X, uint16_t* __restrict__ Xt, float* __restrict__ T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * kFeat) return;
+  const int b = i / kFeat;
+  const int c = i % kFeat;
+  const int64_t src = idx[(int64_t)(*iter) * B + b];
+  const uint16_t h = feat[src * kFeat + c];
+  X[i] = h;
+  Xt[(size_t)c * B + b] = h;
+  if (c < 2) T[b * 2 + c] = tgt[src * 2 + c];
+}
+
+// Mean-squared error over n_valid rows and the 2 outputs, L = sum e^2 / (2 n);
+// dL/dY = e / n as bf16 in both layouts (padding rows and columns 0). One
+// workgroup; fixed reduction order.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, const float* __restrict__ T, int rows,
+                                              int n_valid, uint16_t* __restrict__ dY, uint16_t* __restrict__ dYt,
+                                              float* __restrict__ loss, const int32_t* __restrict__ iter) {
+  __shared__ float red[256];
+  const float inv_n = 1.0f / (float)n_valid;
+  float acc = 0.f;
+  for (int i = threadIdx.x; i < rows * kOutPad; i += 256) {
+    const int b = i / kOutPad;
+    const int c = i % kOutPad;
+    float d = 0.f;
+    if (b < n_valid && c < 2) {
+      const float err = Y[i] - T[b * 2 + c];
+      acc += err * err;
+      d = err * inv_n;
+    }
+    const uint16_t h = f2bf(d);
+    dY[i] = h;
+    dYt[(size_t)c * rows + b] = h;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[iter ? *iter : 0] = 0.5f * red[0] * inv_n;
+}
+
+// out[r] = sum_c M[r][c] (bf16 rows of length cols), one wavefront per row,
+// fixed order: lane-strided partial sums, then a butterfly.
+__global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ Mt, int cols, int nrows,
+                                                float* __restrict__ out) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const uint16_t* p = Mt + (size_t)row * cols;
+  float s = 0.f;
+  for (int c = lane; c < cols; c += 64) s += bf2f(p[c]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[row] = s;
+}
+
+struct MatDesc {
+  int64_t off;
+  int n, k;
+  uint16_t* w;   // [n x k]
+  uint16_t* wt;  // [k x n]
+};
+struct MatTable {
+  MatDesc m[kLayers];
+};
+
+// Adam (the update order of torch.optim.Adam: bias-corrected step size
+// lr / (1 - b1^t), denominator sqrt(v) / sqrt(1 - b2^t) + eps) on the flat fp32
+// parameters, then the bf16 copies of the weight matrices in both layouts.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t P, const int32_t* __restrict__ step, float lr, float b1,
+                       float b2, float eps, MatTable mats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float t = (float)(*step + 1);
+  const float bc1 = 1.0f - powf(b1, t);
+  const float bc2 = 1.0f - powf(b2, t);
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float denom = sqrtf(vi) / sqrtf(bc2) + eps;
+  const float pi = p[i] - (lr / bc1) * (mi / denom);
+  p[i] = pi;
+  for (int l = 0; l < kLayers; ++l) {
+    const MatDesc& d = mats.m[l];
+    const int64_t j = i - d.off;
+    if (j >= 0 && j < (int64_t)d.n * d.k) {
+      const int r = (int)(j / d.k);
+      const int c = (int)(j % d.k);
+      const uint16_t h = f2bf(pi);
+      d.w[j] = h;
+      d.wt[(size_t)c * d.n + r] = h;
+    }
+  }
+}
+
+// bf16 copies of the weights without an update (after set_params).
+__global__ void k_weights_bf16(const float* __restrict__ p, int64_t P, MatTable mats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  for (int l = 0; l < kLayers; ++l) {
+    const MatDesc& d = mats.m[l];
+    const int64_t j = i - d.off;
+    if (j >= 0 && j < (int64_t)d.n * d.k) {
+      const int r = (int)(j / d.k);
+      const int c = (int)(j % d.k);
+      const uint16_t h = f2bf(p[i]);
+      d.w[j] = h;
+      d.wt[(size_t)c * d.n + r] = h;
+    }
+  }
+}
+
+__global__ void k_counters(int32_t* step, int32_t* iter, int reset_iter) {
+  if (threadIdx.x == 0) {
+    if (reset_iter) {
+      *iter = 0;
+    } else {
+      *step += 1;
+      *iter += 1;
+    }
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_zero_f32(float* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// ------------------------------------------------------ host encoding
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+void encode_row(const double* x, float* f) {
+  const double U = x[0], V = x[1], tp = x[2], px = x[3], py = x[4], beta = x[5], mu = x[6];
+  double e[12];
+  e[0] = (U - 7.0) / 3.0;
+  e[1] = V - 1.0;
+  e[2] = tp / 0.3;
+  e[3] = std::sin(kTwoPi * px);
+  e[4] = std::cos(kTwoPi * px);
+  e[5] = std::sin(kTwoPi * py);
+  e[6] = std::cos(kTwoPi * py);
+  e[7] = (beta - 11.0) / 9.0;
+  e[8] = (1.0 / beta - 0.275) / 0.225;
+  e[9] = (mu - 5.0) / 10.0;
+  e[10] = (mu - 0.5 * U) / 5.0;
+  e[11] = beta * (mu - 0.5 * U) / 100.0;
+  for (int i = 0; i < kFeat; ++i) f[i] = i < 12 ? (float)e[i] : 0.0f;
+}
+
+}  // namespace
+
+// ============================================================ the object
+struct ftlm_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ftlm_mlp_config cfg{};
+  int64_t cap = 0;  // rows the activation buffers hold
+  Buf params, grads, adam_m, adam_v;
+  Buf wb[kLayers], wt[kLayers];
+  Buf x0, x0t, h[kNHid + 1], ht[kNHid + 1], z[kNHid], y, t, dy, dyt, dz[2], dzt[2];
+  Buf feat_host_dev;  // fp32 feature staging
+  Buf data_feat, data_tgt, idx, loss, counters;
+  int64_t n_data = 0;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> exec_key;  // what the graph baked in (buffers, batch, rate)
+  int exec_steps = 0;
+  bool params_set = false;
+
+  ~ftlm_mlp() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  MatTable mats() {
+    MatTable t{};
+    for (int l = 0; l < kLayers; ++l)
+      t.m[l] = MatDesc{w_off(l), layer_out(l), layer_in(l), wb[l].as<uint16_t>(), wt[l].as<uint16_t>()};
+    return t;
+  }
+
+  void ensure_rows(int64_t rows) {
+    if (rows <= cap) return;
+    x0.get<uint16_t>(rows * kFeat);
+    x0t.get<uint16_t>(rows * kFeat);
+    for (int j = 1; j <= kNHid; ++j) {
+      h[j].get<uint16_t>(rows * kHid);
+      ht[j].get<uint16_t>(rows * kHid);
+    }
+    for (int j = 0; j < kNHid; ++j) z[j].get<float>(rows * kHid);
+    y.get<float>(rows * kOutPad);
+    t.get<float>(rows * 2);
+    dy.get<uint16_t>(rows * kOutPad);
+    dyt.get<uint16_t>(rows * kOutPad);
+    for (int i = 0; i < 2; ++i) {
+      dz[i].get<uint16_t>(rows * kHid);
+      dzt[i].get<uint16_t>(rows * kHid);
+    }
+    cap = rows;
+  }
+
+  // forward over `rows` rows of x0 (train: keep pre-activations and
+  // transposed activations for the backward pass)
+  void forward(int rows, bool train) {
+    const uint16_t* in = x0.as<uint16_t>();
+    int k = kFeat;
+    for (int j = 0; j < kNHid; ++j) {
+      Epi e{};
+      e.bias = params.as<float>() + b_off(j);
+      e.zf = train ? z[j].as<float>() : nullptr;
+      e.ldz = kHid;
+      e.hb = h[j + 1].as<uint16_t>();
+      e.ldh = kHid;
+      e.ht = train ? ht[j + 1].as<uint16_t>() : nullptr;
+      e.ldt = rows;
+      gemm<EPI_GELU_FWD>(in, k, wb[j].as<uint16_t>(), k, rows, kHid, k, e, stream);
+      in = h[j + 1].as<uint16_t>();
+      k = kHid;
+    }
+    Epi e{};
+    e.bias = params.as<float>() + b_off(kNHid);
+    e.zf = y.as<float>();
+    e.ldz = kOutPad;
+    gemm<EPI_OUT>(in, kHid, wb[kNHid].as<uint16_t>(), kHid, rows, kOutPad, kHid, e, stream);
+  }
+
+  // loss + backward into grads (rows with index >= n_valid contribute 0)
+  void backward(int rows, int n_valid, float* loss_out, const int32_t* iter) {
+    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, stream, y.as<float>(), t.as<float>(), rows, n_valid,
+                       dy.as<uint16_t>(), dyt.as<uint16_t>(), loss_out, iter);
+    MLPCHECK(hipGetLastError());
+    float* g = grads.as<float>();
+    // output layer: dW_4 = dY^T H_4, db_4 = colsum dY, dH_4 = dY W_4 -> dZ_3
+    {
+      Epi e{};
+      e.zf = g + w_off(kNHid);
+      e.ldz = kHid;
+      gemm<EPI_F32>(dyt.as<uint16_t>(), rows, ht[kNHid].as<uint16_t>(), rows, kOutPad, kHid, rows, e, stream);
+      hipLaunchKernelGGL(k_rowsum, dim3(blocks_for(kOutPad, 4)), dim3(256), 0, stream, dyt.as<uint16_t>(), rows,
+                         kOutPad, g + b_off(kNHid));
+      MLPCHECK(hipGetLastError());
+      Epi d{};
+      d.zf = z[kNHid - 1].as<float>();
+      d.ldz = kHid;
+      d.hb = dz[1].as<uint16_t>();
+      d.ldh = kHid;
+      d.ht = dzt[1].as<uint16_t>();
+      d.ldt = rows;
+      gemm<EPI_GELU_BWD>(dy.as<uint16_t>(), kOutPad, wt[kNHid].as<uint16_t>(), kOutPad, rows, kHid, kOutPad, d,
+                         stream);
+    }
+    int cur = 1;
+    for (int j = kNHid - 1; j >= 0; --j) {
+      const int kin = layer_in(j);
+      const uint16_t* hin_t = j == 0 ? x0t.as<uint16_t>() : ht[j].as<uint16_t>();
+      Epi e{};
+      e.zf = g + w_off(j);
+      e.ldz = kin;
+      gemm<EPI_F32>(dzt[cur].as<uint16_t>(), rows, hin_t, rows, kHid, kin, rows, e, stream);
+      hipLaunchKernelGGL(k_rowsum, dim3(blocks_for(kHid, 4)), dim3(256), 0, stream, dzt[cur].as<uint16_t>(), rows,
+                         kHid, g + b_off(j));
+      MLPCHECK(hipGetLastError());
+      if (j > 0) {
+        Epi d{};
+        d.zf = z[j - 1].as<float>();
+        d.ldz = kHid;
+        d.hb = dz[cur ^ 1].as<uint16_t>();
+        d.ldh = kHid;
+        d.ht = dzt[cur ^ 1].as<uint16_t>();
+        d.ldt = rows;
+        gemm<EPI_GELU_BWD>(dz[cur].as<uint16_t>(), kHid, wt[j].as<uint16_t>(), kHid, rows, kHid, kHid, d, stream);
+        cur ^= 1;
+      }
+    }
+  }
+
+  void adam() {
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(kParams)), dim3(256), 0, stream, params.as<float>(),
+                       grads.as<float>(), adam_m.as<float>(), adam_v.as<float>(), kParams,
+                       counters.as<int32_t>(), cfg.lr, cfg.beta1, cfg.beta2, cfg.eps, mats());
+    MLPCHECK(hipGetLastError());
+  }
+
+  // rows of raw inputs -> x0 (+ x0t), padded with zero rows up to `rows`
+  void upload_inputs(int64_t n, const double* x, int64_t rows) {
+    std::vector<float> f((size_t)n * kFeat);
+    for (int64_t i = 0; i < n; ++i) encode_row(x + 7 * i, f.data() + kFeat * i);
+    float* d = feat_host_dev.get<float>((size_t)std::max<int64_t>(n, 1) * kFeat);
+    MLPCHECK(hipMemcpyAsync(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_feat_bf16, dim3(blocks_for(rows * kFeat)), dim3(256), 0, stream, d, n, rows,
+                       x0.as<uint16_t>(), x0t.as<uint16_t>());
+    MLPCHECK(hipGetLastError());
+    sync();  // the host staging vector ends here
+  }
+
+  void upload_targets(int64_t n, const double* yv, int64_t rows, float* dst) {
+    std::vector<float> tv((size_t)rows * 2, 0.0f);
+    for (int64_t i = 0; i < n; ++i)
+      for (int c = 0; c < 2; ++c) tv[2 * i + c] = (float)((yv[2 * i + c] - cfg.y_mean[c]) / cfg.y_scale[c]);
+    MLPCHECK(hipMemcpyAsync(dst, tv.data(), tv.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    sync();
+  }
+
+  void sync() { MLPCHECK(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+void check_mlp(ftlm_mlp* m) {
+  if (!m) throw Error(FTLM_ERR_ARG, "null surrogate");
+  MLPCHECK(hipSetDevice(m->device));
+  if (!m->params_set) throw Error(FTLM_ERR_STATE, "surrogate parameters not set (call ftlm_mlp_set_params)");
+}
+
+int64_t pad64(int64_t n) { return (n + 63) / 64 * 64; }
+
+void check_rows(const ftlm_mlp* m, int64_t n) {
+  if (n <= 0) throw Error(FTLM_ERR_ARG, "row count must be positive");
+  if (pad64(n) > std::max<int64_t>(m->cfg.batch, 1 << 16))
+    throw Error(FTLM_ERR_LIMIT, "too many rows for one gradient evaluation");
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ftlm_mlp_param_count(void) { return kParams; }
+
+int ftlm_mlp_layout(int64_t* w_offsets, int64_t* b_offsets, int32_t* n_out, int32_t* n_in) {
+  return guarded([&] {
+    for (int l = 0; l < kLayers; ++l) {
+      if (w_offsets) w_offsets[l] = w_off(l);
+      if (b_offsets) b_offsets[l] = b_off(l);
+      if (n_out) n_out[l] = layer_out(l);
+      if (n_in) n_in[l] = layer_in(l);
+    }
+  });
+}
+
+int ftlm_mlp_encode(int64_t n, const double* x, float* feat) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!x || !feat))) throw Error(FTLM_ERR_ARG, "null argument");
+    for (int64_t i = 0; i < n; ++i) encode_row(x + 7 * i, feat + kFeat * i);
+  });
+}
+
+int ftlm_mlp_create(const ftlm_mlp_config* cfg, int device, ftlm_mlp** out) {
+  return guarded([&] {
+    if (!cfg || !out) throw Error(FTLM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->batch <= 0 || cfg->batch % 64) throw Error(FTLM_ERR_ARG, "batch must be a positive multiple of 64");
+    if (cfg->batch > (1 << 20)) throw Error(FTLM_ERR_LIMIT, "batch too large");
+    if (!(cfg->y_scale[0] > 0.0) || !(cfg->y_scale[1] > 0.0))
+      throw Error(FTLM_ERR_ARG, "target scales must be positive");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+      throw Error(FTLM_ERR_DEVICE, "no HIP device available (the FTLM engine runs on the GPU only)");
+    if (device < 0 || device >= n) throw Error(FTLM_ERR_DEVICE, "device index out of range");
+    auto m = std::make_unique<ftlm_mlp>();
+    m->device = device;
+    m->cfg = *cfg;
+    MLPCHECK(hipSetDevice(device));
+    MLPCHECK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    m->params.get<float>(kParams);
+    m->grads.get<float>(kParams);
+    m->adam_m.get<float>(kParams);
+    m->adam_v.get<float>(kParams);
+    for (int l = 0; l < kLayers; ++l) {
+      m->wb[l].get<uint16_t>((size_t)layer_out(l) * layer_in(l));
+      m->wt[l].get<uint16_t>((size_t)layer_out(l) * layer_in(l));
+    }
+    m->counters.get<int32_t>(2);
+    m->ensure_rows(cfg->batch);
+    *out = m.release();
+  });
+}
+
+void ftlm_mlp_destroy(ftlm_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  delete m;
+}
+
+int ftlm_mlp_set_params(ftlm_mlp* m, const float* p) {
+  return guarded([&] {
+    if (!m || !p) throw Error(FTLM_ERR_ARG, "null argument");
+    MLPCHECK(hipSetDevice(m->device));
+    MLPCHECK(hipMemcpyAsync(m->params.as<float>(), p, kParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks_for(kParams)), dim3(256), 0, m->stream, m->adam_m.as<float>(),
+                       kParams);
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks_for(kParams)), dim3(256), 0, m->stream, m->adam_v.as<float>(),
+                       kParams);
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks_for(kParams)), dim3(256), 0, m->stream, m->grads.as<float>(),
+                       kParams);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, m->stream, m->counters.as<int32_t>(), 2);
+    hipLaunchKernelGGL(k_weights_bf16, dim3(blocks_for(kParams)), dim3(256), 0, m->stream, m->params.as<float>(),
+                       kParams, m->mats());
+    MLPCHECK(hipGetLastError());
+    m->sync();
+    m->params_set = true;
+  });
+}
+
+int ftlm_mlp_get_params(ftlm_mlp* m, float* p) {
+  return guarded([&] {
+    check_mlp(m);
+    if (!p) throw Error(FTLM_ERR_ARG, "null argument");
+    MLPCHECK(hipMemcpyAsync(p, m->params.as<float>(), kParams * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+  });
+}
+
+int ftlm_mlp_set_lr(ftlm_mlp* m, float lr) {
+  return guarded([&] {
+    if (!m) throw Error(FTLM_ERR_ARG, "null surrogate");
+    if (!(lr >= 0.0f)) throw Error(FTLM_ERR_ARG, "learning rate must be non-negative");
+    m->cfg.lr = lr;  // a kernel argument: the training graph is rebuilt on its next use
+  });
+}
+
+int ftlm_mlp_set_data(ftlm_mlp* m, int64_t n, const double* x, const double* y) {
+  return guarded([&] {
+    if (!m || !x || !y) throw Error(FTLM_ERR_ARG, "null argument");
+    if (n <= 0) throw Error(FTLM_ERR_ARG, "empty training set");
+    if (n > (int64_t)1 << 30) throw Error(FTLM_ERR_LIMIT, "training set too large");
+    MLPCHECK(hipSetDevice(m->device));
+    std::vector<float> f((size_t)n * kFeat);
+    for (int64_t i = 0; i < n; ++i) encode_row(x + 7 * i, f.data() + kFeat * i);
+    float* d = m->feat_host_dev.get<float>((size_t)n * kFeat);
+    MLPCHECK(hipMemcpyAsync(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    uint16_t* df = m->data_feat.get<uint16_t>((size_t)n * kFeat);
+    hipLaunchKernelGGL(k_feat_bf16, dim3(blocks_for(n * kFeat)), dim3(256), 0, m->stream, d, n, n, df,
+                       (uint16_t*)nullptr);
+    MLPCHECK(hipGetLastError());
+    float* dt = m->data_tgt.get<float>((size_t)n * 2);
+    m->upload_targets(n, y, n, dt);
+    m->sync();
+    m->n_data = n;
+  });
+}
+
+int ftlm_mlp_train(ftlm_mlp* m, int steps, uint64_t seed, float* loss) {
+  return guarded([&] {
+    check_mlp(m);
+    if (steps <= 0) throw Error(FTLM_ERR_ARG, "steps must be positive");
+    if (m->n_data <= 0) throw Error(FTLM_ERR_STATE, "no training data (call ftlm_mlp_set_data)");
+    const int B = m->cfg.batch;
+    // batches: consecutive slices of seeded random permutations (epochs)
+    std::vector<int32_t> order((size_t)steps * B);
+    std::vector<int32_t> perm((size_t)m->n_data);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    size_t pos = perm.size();
+    for (size_t i = 0; i < order.size(); ++i) {
+      if (pos == perm.size()) {
+        for (size_t j = perm.size() - 1; j > 0; --j) {
+          const size_t k = (size_t)(rng() % (j + 1));
+          std::swap(perm[j], perm[k]);
+        }
+        pos = 0;
+      }
+      order[i] = perm[pos++];
+    }
+    int32_t* didx = m->idx.get<int32_t>(order.size());
+    float* dloss = m->loss.get<float>((size_t)steps);
+    MLPCHECK(hipMemcpyAsync(didx, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
+    int32_t* step = m->counters.as<int32_t>();
+    int32_t* iter = step + 1;
+    hipLaunchKernelGGL(k_counters, dim3(1), dim3(64), 0, m->stream, step, iter, 1);
+    MLPCHECK(hipGetLastError());
+    uint32_t lr_bits;
+    memcpy(&lr_bits, &m->cfg.lr, 4);
+    const std::vector<uint64_t> key = {(uint64_t)B, (uint64_t)lr_bits, (uint64_t)m->cap,
+                                       (uint64_t)(uintptr_t)didx, (uint64_t)(uintptr_t)dloss,
+                                       (uint64_t)(uintptr_t)m->data_feat.p, (uint64_t)(uintptr_t)m->data_tgt.p,
+                                       (uint64_t)(uintptr_t)m->x0.p};
+    if (m->exec && key != m->exec_key) {
+      MLPCHECK(hipGraphExecDestroy(m->exec));
+      m->exec = nullptr;
+    }
+    if (!m->exec) {
+      hipGraph_t graph = nullptr;
+      MLPCHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_gather, dim3(blocks_for((int64_t)B * kFeat)), dim3(256), 0, m->stream,
+                         m->data_feat.as<uint16_t>(), m->data_tgt.as<float>(), m->idx.as<int32_t>(), iter, B,
+                         m->x0.as<uint16_t>(), m->x0t.as<uint16_t>(), m->t.as<float>());
+      m->forward(B, true);
+      m->backward(B, B, m->loss.as<float>(), iter);
+      m->adam();
+      hipLaunchKernelGGL(k_counters, dim3(1), dim3(64), 0, m->stream, step, iter, 0);
+      const hipError_t ce = hipStreamEndCapture(m->stream, &graph);
+      if (ce != hipSuccess)
+        throw Error(FTLM_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&m->exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        m->exec = nullptr;
+        throw Error(FTLM_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(ie));
+      }
+      m->exec_key = key;
+      m->exec_steps = steps;
+    }
+    for (int s = 0; s < steps; ++s) MLPCHECK(hipGraphLaunch(m->exec, m->stream));
+    if (loss) MLPCHECK(hipMemcpyAsync(loss, dloss, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+  });
+}
+
+int ftlm_mlp_gradients(ftlm_mlp* m, int64_t n, const double* x, const double* y, float* grad, float* loss) {
+  return guarded([&] {
+    check_mlp(m);
+    if (!x || !y) throw Error(FTLM_ERR_ARG, "null argument");
+    check_rows(m, n);
+    const int64_t rows = pad64(n);
+    m->ensure_rows(rows);
+    m->upload_inputs(n, x, rows);
+    m->upload_targets(n, y, rows, m->t.as<float>());
+    m->forward((int)rows, true);
+    float* dl = m->loss.get<float>(std::max(m->exec_steps, 1));
+    m->backward((int)rows, (int)n, dl, nullptr);
+    if (grad) MLPCHECK(hipMemcpyAsync(grad, m->grads.as<float>(), kParams * sizeof(float), hipMemcpyDeviceToHost,
+                                      m->stream));
+    if (loss) MLPCHECK(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+  });
+}
+
+int ftlm_mlp_step(ftlm_mlp* m, int64_t n, const double* x, const double* y, float* loss) {
+  return guarded([&] {
+    check_mlp(m);
+    if (!x || !y) throw Error(FTLM_ERR_ARG, "null argument");
+    check_rows(m, n);
+    const int64_t rows = pad64(n);
+    m->ensure_rows(rows);
+    m->upload_inputs(n, x, rows);
+    m->upload_targets(n, y, rows, m->t.as<float>());
+    m->forward((int)rows, true);
+    float* dl = m->loss.get<float>(std::max(m->exec_steps, 1));
+    m->backward((int)rows, (int)n, dl, nullptr);
+    m->adam();
+    int32_t* step = m->counters.as<int32_t>();
+    hipLaunchKernelGGL(k_counters, dim3(1), dim3(64), 0, m->stream, step, step + 1, 0);
+    MLPCHECK(hipGetLastError());
+    if (loss) MLPCHECK(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+  });
+}
+
+int ftlm_mlp_predict(ftlm_mlp* m, int64_t n, const double* x, double* y) {
+  return guarded([&] {
+    check_mlp(m);
+    if (n < 0 || (n > 0 && (!x || !y))) throw Error(FTLM_ERR_ARG, "null argument");
+    const int64_t chunk = std::max<int64_t>(m->cap, 64) / 64 * 64;
+    std::vector<float> out((size_t)chunk * kOutPad);
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+      const int64_t cnt = std::min(chunk, n - r0);
+      const int64_t rows = pad64(cnt);
+      m->upload_inputs(cnt, x + 7 * r0, rows);
+      m->forward((int)rows, false);
+      MLPCHECK(hipMemcpyAsync(out.data(), m->y.as<float>(), (size_t)rows * kOutPad * sizeof(float),
+                              hipMemcpyDeviceToHost, m->stream));
+      m->sync();
+      for (int64_t i = 0; i < cnt; ++i)
+        for (int c = 0; c < 2; ++c)
+          y[2 * (r0 + i) + c] = m->cfg.y_mean[c] + m->cfg.y_scale[c] * (double)out[(size_t)i * kOutPad + c];
+    }
+  });
+}
+
+int ftlm_mlp_gemm(ftlm_mlp* m, int M, int N, int K, const float* a, const float* b, float* c) {
+  return guarded([&] {
+    if (!m || !a || !b || !c) throw Error(FTLM_ERR_ARG, "null argument");
+    if (M <= 0 || N <= 0 || K <= 0 || M % 16 || N % 16 || K % 32 || M > 4096 || N > 4096 || K > 4096)
+      throw Error(FTLM_ERR_ARG, "GEMM test shapes: M, N multiples of 16, K a multiple of 32, each <= 4096");
+    MLPCHECK(hipSetDevice(m->device));
+    Buf fa, fb, ba, bb, fc;
+    float* da = fa.get<float>((size_t)M * K);
+    float* db = fb.get<float>((size_t)N * K);
+    MLPCHECK(hipMemcpyAsync(da, a, (size_t)M * K * 4, hipMemcpyHostToDevice, m->stream));
+    MLPCHECK(hipMemcpyAsync(db, b, (size_t)N * K * 4, hipMemcpyHostToDevice, m->stream));
+    uint16_t* ha = ba.get<uint16_t>((size_t)M * K);
+    uint16_t* hb = bb.get<uint16_t>((size_t)N * K);
+    // reuse the feature converter: K-wide rows are kFeat-wide chunks
+    hipLaunchKernelGGL(k_feat_bf16, dim3(blocks_for((int64_t)M * K)), dim3(256), 0, m->stream, da,
+                       (int64_t)M * K / kFeat, (int64_t)M * K / kFeat, ha, (uint16_t*)nullptr);
+    hipLaunchKernelGGL(k_feat_bf16, dim3(blocks_for((int64_t)N * K)), dim3(256), 0, m->stream, db,
+                       (int64_t)N * K / kFeat, (int64_t)N * K / kFeat, hb, (uint16_t*)nullptr);
+    MLPCHECK(hipGetLastError());
+    float* dc = fc.get<float>((size_t)M * N);
+    Epi e{};
+    e.zf = dc;
+    e.ldz = N;
+    gemm<EPI_F32>(ha, K, hb, K, M, N, K, e, m->stream);
+    MLPCHECK(hipMemcpyAsync(c, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+  });
+}
+
+}  // extern "C"

@@ -103,6 +103,7 @@ void print_help(const char* argv0) {
             << "  --pack-budget-gb G    device memory per pack in GiB (default 64)\n"
             << "  --twist-grid N        run the twists phix, phiy in {0, 1/N, ..., (N-1)/N} together;\n"
             << "                        one CSV per twist, OUTPUT with .phixA_phiyB before the extension\n"
+            << "  --twist-list L        like --twist-grid for the listed twists PX:PY,PX:PY,...\n"
             << "  --help                show this message\n";
 }
 
@@ -158,6 +159,17 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
     else if (arg == "--pack-wg") p.pack_wg = parse_as<int>(next(arg));
     else if (arg == "--pack-budget-gb") p.pack_budget_gb = parse_as<double>(next(arg));
     else if (arg == "--twist-grid") p.twist_grid = parse_as<int>(next(arg));
+    else if (arg == "--twist-list") {
+      std::stringstream ss(next(arg));
+      std::string item;
+      p.twist_list.clear();
+      while (std::getline(ss, item, ',')) {
+        const std::size_t colon = item.find(':');
+        if (colon == std::string::npos) die("--twist-list entries must be PX:PY, got: " + item);
+        p.twist_list.push_back({parse_as<double>(item.substr(0, colon)), parse_as<double>(item.substr(colon + 1))});
+      }
+      if (p.twist_list.empty()) die("--twist-list must name at least one twist.");
+    }
     else die("Unknown argument: " + arg);
   }
   if (p.lx <= 0 || p.ly <= 0) die("Lattice dimensions must be positive.");
@@ -175,6 +187,7 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
     die("--pack-width must be 4, 8, 16, 32 or 64.");
   if (p.pack_wg < 0 || p.pack_budget_gb < 0.0) die("--pack-wg and --pack-budget-gb must be non-negative.");
   if (p.twist_grid < 0) die("--twist-grid must be non-negative.");
+  if (p.twist_grid > 0 && !p.twist_list.empty()) die("--twist-grid and --twist-list are exclusive.");
   const bool partial = p.dbg_nup >= 0 || p.dbg_ndn >= 0 || p.dbg_mx >= 0 || p.dbg_my >= 0 || !p.debug_csv.empty();
   const bool full = p.dbg_nup >= 0 && p.dbg_ndn >= 0 && p.dbg_mx >= 0 && p.dbg_my >= 0;
   if (partial && !full)

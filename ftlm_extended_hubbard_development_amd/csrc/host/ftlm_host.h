@@ -38,6 +38,8 @@ struct Options {
   double pack_budget_gb = 0.0;      // device bytes per pack (0: engine default)
   // twist scan: phix, phiy in {0, 1/N, ..., (N-1)/N}, one CSV per twist
   int twist_grid = 0;
+  // twist scan over listed (phix, phiy) pairs, one CSV per twist
+  std::vector<std::pair<double, double>> twist_list;
 };
 
 // parse_args (main.cpp:257-368). Returns false when --help was printed.

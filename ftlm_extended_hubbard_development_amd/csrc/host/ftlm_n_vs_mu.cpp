@@ -61,9 +61,13 @@ struct Item {
   bool ready = false;
 };
 
-std::string twist_output(const std::string& out, double px, double py) {
+// OUTPUT with a twist tag before the extension: .phixA_phiyB for grid twists,
+// .twistI (position in the list) for --twist-list, whose values may not print
+// distinctly.
+std::string twist_output(const std::string& out, double px, double py, int list_index) {
   std::ostringstream tag;
-  tag << ".phix" << px << "_phiy" << py;
+  if (list_index >= 0) tag << ".twist" << list_index;
+  else tag << ".phix" << px << "_phiy" << py;
   const std::size_t slash = out.find_last_of('/');
   const std::size_t dot = out.find_last_of('.');
   if (dot == std::string::npos || (slash != std::string::npos && dot < slash)) return out + tag.str() + ".csv";
@@ -83,14 +87,16 @@ int main(int argc, char** argv) {
     const bool dbg = o.dbg_nup >= 0 && o.dbg_ndn >= 0 && o.dbg_mx >= 0 && o.dbg_my >= 0;
     if (!o.checkpoint.empty() && (trace || dbg))
       die("--checkpoint is not currently compatible with trace/debug block output.");
-    const bool scan = o.twist_grid > 0;
+    const bool scan = o.twist_grid > 0 || !o.twist_list.empty();
     if (scan && (!o.checkpoint.empty() || trace || dbg))
-      die("--twist-grid is not compatible with --checkpoint or trace/debug block output.");
+      die("--twist-grid/--twist-list are not compatible with --checkpoint or trace/debug block output.");
     std::vector<std::pair<double, double>> twists;
-    if (scan) {
+    if (o.twist_grid > 0) {
       for (int iy = 0; iy < o.twist_grid; ++iy)
         for (int ix = 0; ix < o.twist_grid; ++ix)
           twists.push_back({(double)ix / o.twist_grid, (double)iy / o.twist_grid});
+    } else if (scan) {
+      twists = o.twist_list;
     } else {
       twists.push_back({o.phix, o.phiy});
     }
@@ -112,7 +118,10 @@ int main(int argc, char** argv) {
     std::cout << " U=" << o.u << " V=" << o.v << " tx=" << o.tx << " ty=" << o.ty << " tp=" << o.tp
               << " phix=" << o.phix << " phiy=" << o.phiy << " threads=" << threads
               << " exact_block_threshold=" << o.threshold << "\n";
-    if (scan) std::cout << "twist grid " << o.twist_grid << "x" << o.twist_grid << " (" << ntw << " twists)\n";
+    if (o.twist_grid > 0)
+      std::cout << "twist grid " << o.twist_grid << "x" << o.twist_grid << " (" << ntw << " twists)\n";
+    else if (scan)
+      std::cout << "twist list (" << ntw << " twists)\n";
     if (!o.checkpoint.empty())
       std::cout << "checkpoint=" << o.checkpoint << " loaded_blocks=" << ckpt.size() << "\n";
 
@@ -438,7 +447,8 @@ int main(int argc, char** argv) {
       die("Selected block debug target was not found among active FTLM blocks.");
 
     for (int t = 0; t < ntw; ++t) {
-      const std::string out = scan ? twist_output(o.output, twists[t].first, twists[t].second) : o.output;
+      const std::string out =
+          scan ? twist_output(o.output, twists[t].first, twists[t].second, o.twist_list.empty() ? -1 : t) : o.output;
       if (scan) std::cout << "twist phix=" << twists[t].first << " phiy=" << twists[t].second << "\n";
       if (multi) {
         std::vector<Results> all;

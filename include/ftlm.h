@@ -185,6 +185,57 @@ int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes);
 int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t* my, double* eig_out,
                       int64_t* offsets);
 
+/* ---- Learned surrogate (no reference counterpart) -------------------------
+ * An MLP maps (U, V, t', phix, phiy, beta, mu) to (n, compressibility), the
+ * two observables the reference's CSV reports per mu (main.cpp:1409-1455),
+ * so that dense parameter and twist scans can be filled between FTLM runs;
+ * FTLM stays the judge and the fallback. Inputs are rows of 7 doubles in that
+ * order, outputs rows of 2 doubles. The network: 12 encoded features (zero
+ * padded to FTLM_MLP_FEATURES) -> 4 hidden layers of 256 GELU units -> 2
+ * outputs (padded to 32), output c = y_mean[c] + y_scale[c] * raw output.
+ * Weights are bf16 copies of fp32 master parameters; every product runs on
+ * the MFMA units (bf16 x bf16, fp32 accumulation). Training minimises the
+ * mean squared error of the normalised outputs with Adam.
+ *   ftlm_mlp_param_count / ftlm_mlp_layout  flat fp32 parameter vector:
+ *                     W_0, b_0, ..., W_4, b_4, W_l row-major [n_out x n_in]
+ *   ftlm_mlp_encode   the feature rows (host only, no device needed)
+ *   ftlm_mlp_create   needs a GPU; then ftlm_mlp_set_params before use
+ *                     (also clears the Adam state and step counter)
+ *   ftlm_mlp_set_data uploads a training set (resident on the device)
+ *   ftlm_mlp_train    `steps` Adam steps over seeded random batches of
+ *                     cfg.batch rows (epochs of random permutations), one
+ *                     HIP graph replay per step; loss[steps] receives each
+ *                     step's loss (may be NULL)
+ *   ftlm_mlp_gradients  gradient (flat, like the parameters) and loss of n
+ *                     rows without updating
+ *   ftlm_mlp_step     one Adam step on n given rows
+ *   ftlm_mlp_predict  n rows of (n, compressibility)
+ *   ftlm_mlp_gemm     test hook: C[M x N] = bf16(A[M x K]) bf16(B[N x K])^T in
+ *                     fp32 through the network's MFMA GEMM kernel */
+#define FTLM_MLP_INPUTS 7
+#define FTLM_MLP_FEATURES 32
+#define FTLM_MLP_LAYERS 5
+typedef struct ftlm_mlp ftlm_mlp;
+typedef struct ftlm_mlp_config {
+  int32_t batch;                 /* rows per training step, a multiple of 64 */
+  float lr, beta1, beta2, eps;   /* Adam */
+  double y_mean[2], y_scale[2];  /* output normalisation */
+} ftlm_mlp_config;
+int64_t ftlm_mlp_param_count(void);
+int ftlm_mlp_layout(int64_t* w_offsets, int64_t* b_offsets, int32_t* n_out, int32_t* n_in);
+int ftlm_mlp_encode(int64_t n, const double* x, float* feat);
+int ftlm_mlp_create(const ftlm_mlp_config* cfg, int device, ftlm_mlp** out);
+void ftlm_mlp_destroy(ftlm_mlp* m);
+int ftlm_mlp_set_params(ftlm_mlp* m, const float* params);
+int ftlm_mlp_get_params(ftlm_mlp* m, float* params);
+int ftlm_mlp_set_lr(ftlm_mlp* m, float lr);
+int ftlm_mlp_set_data(ftlm_mlp* m, int64_t n, const double* x, const double* y);
+int ftlm_mlp_train(ftlm_mlp* m, int steps, uint64_t seed, float* loss);
+int ftlm_mlp_gradients(ftlm_mlp* m, int64_t n, const double* x, const double* y, float* grad, float* loss);
+int ftlm_mlp_step(ftlm_mlp* m, int64_t n, const double* x, const double* y, float* loss);
+int ftlm_mlp_predict(ftlm_mlp* m, int64_t n, const double* x, double* y);
+int ftlm_mlp_gemm(ftlm_mlp* m, int M, int N, int K, const float* a, const float* b, float* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,171 @@
+"""GPU surrogate kernels (ftlm_mlp_* of libftlm_hip.so) against the CPU
+oracle oracle/mlp_oracle.py.
+
+* The MFMA GEMM kernel against float64 products of the same bf16-rounded
+  operands: the products of two bf16 numbers are exact in fp32, so the only
+  difference is fp32 accumulation (relative error ~ K * 2^-24).
+* Forward pass and gradients against the oracle with bf16 rounding emulated
+  at the points where the kernels store bf16. What remains is fp32 versus
+  float64 accumulation, which now and then moves a stored bf16 value by one
+  bf16 unit (2^-8 relative); the tolerances below bound that.
+* Adam: the device update against the oracle's torch.optim.Adam formula
+  applied to the device's own gradient (fp32 versus float64 arithmetic).
+* Training through the replayed HIP graph equals the same number of eager
+  steps; the loss of a smooth target falls.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sur():
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    s = surrogate.Surrogate(y_mean=(0.0, 0.0), y_scale=(1.0, 1.0), batch=256, lr=1e-3, seed=1)
+    yield s
+    s.close()
+
+
+def rand_inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    lo = np.array([4.0, 0.0, -0.3, 0.0, 0.0, 2.0, -5.0])
+    hi = np.array([10.0, 2.0, 0.3, 1.0, 1.0, 20.0, 15.0])
+    return lo + (hi - lo) * rng.random((n, 7))
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (64, 64, 32), (128, 256, 256), (48, 80, 96), (256, 32, 4096)])
+def test_mfma_gemm(sur, M, N, K):
+    say(f"gemm M={M} N={N} K={K}")
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    a = rng.standard_normal((M, K)).astype(np.float32)
+    b = rng.standard_normal((N, K)).astype(np.float32)
+    c = sur.gemm(a, b)
+    ref = mo.bf16(a) @ mo.bf16(b).T
+    scale = np.sqrt(K) * np.abs(mo.bf16(a)).max() * np.abs(mo.bf16(b)).max()
+    assert np.abs(c - ref).max() <= 4 * K * 2.0 ** -24 * scale
+    # layout check beyond the tolerance: a single nonzero picks one element
+    a1 = np.zeros((M, K), np.float32)
+    b1 = np.zeros((N, K), np.float32)
+    i, j, k = M - 1, N // 2 + 1 if N > 17 else 1, K - 3
+    a1[i, k] = 1.0
+    b1[j, k] = 2.0
+    c1 = sur.gemm(a1, b1)
+    assert c1[i, j] == 2.0 and np.count_nonzero(c1) == 1
+
+
+def test_forward_matches_oracle(sur):
+    p = sur.params().astype(np.float64)
+    x = rand_inputs(1000, 2)  # not a multiple of 64: padded rows
+    y = sur.predict(x)
+    ref, _ = mo.forward(p, mo.encode(x), emulate=True)
+    err = np.abs(y - ref[:, :2]).max()
+    say(f"forward max |dy| {err:.3e}, max |y| {np.abs(ref[:, :2]).max():.3e}")
+    assert err <= 2e-3 * max(1.0, np.abs(ref[:, :2]).max())
+    assert np.abs(ref[:, 2:]).max() == 0.0  # padded outputs stay zero
+
+
+def test_predict_in_chunks(sur):
+    """More rows than the activation buffers hold: processed in chunks, the
+    same values as row by row."""
+    x = rand_inputs(700, 3)
+    y = sur.predict(x)
+    y2 = np.concatenate([sur.predict(x[i:i + 100]) for i in range(0, 700, 100)])
+    assert np.array_equal(y, y2)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("n", [256, 100])
+def test_gradients_match_oracle(sur, n):
+    p = sur.params().astype(np.float64)
+    x = rand_inputs(n, 4)
+    t = np.random.default_rng(5).standard_normal((n, 2))
+    g, loss = sur.gradients(x, t)
+    lref, gref = mo.loss_and_grad(p, mo.encode(x), t, emulate=True)
+    _, gexact = mo.loss_and_grad(p, mo.encode(x), t, emulate=False)
+    say(f"n={n} loss {loss:.6e} vs {lref:.6e}")
+    assert abs(loss - lref) <= 1e-4 * lref
+    for l in range(5):
+        for name, a in (("W", slice(mo.W_OFF[l], mo.B_OFF[l])),
+                        ("b", slice(mo.B_OFF[l], mo.B_OFF[l] + mo.SHAPES[l][0]))):
+            r_emu = rel(g[a], gref[a])
+            r_ex = rel(g[a], gexact[a])
+            say(f"  layer {l} {name}: rel err vs emulated {r_emu:.2e}, vs float64 {r_ex:.2e}")
+            assert r_emu <= 5e-3, (l, name, r_emu)
+            assert r_ex <= 5e-2, (l, name, r_ex)
+    # padding receives no gradient
+    assert not g[mo.W_OFF[0]:mo.B_OFF[0]].reshape(256, 32)[:, 12:].any()
+    assert not g[mo.W_OFF[4]:mo.B_OFF[4]].reshape(32, 256)[2:, :].any()
+
+
+def test_adam_step_matches_oracle(sur):
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    p0 = surrogate.init_params(3)
+    sur.set_params(p0)  # clears the Adam state and the step counter
+    x = rand_inputs(200, 6)
+    t = np.random.default_rng(7).standard_normal((200, 2))
+    p, m, v = p0.astype(np.float64), np.zeros(p0.size), np.zeros(p0.size)
+    for step in (1, 2, 3):
+        g, _ = sur.gradients(x, t)
+        sur.step(x, t)
+        p1 = sur.params()
+        p, m, v = mo.adam(p, g.astype(np.float64), m, v, step, 1e-3)
+        d = np.abs(p1 - p).max()
+        say(f"adam step {step}: max |dp| {d:.3e}")
+        assert d <= 1e-6
+        p = p1.astype(np.float64)  # continue from the device's parameters
+    sur.set_params(surrogate.init_params(1))
+
+
+def test_graph_training_equals_eager_steps():
+    """ftlm_mlp_train replays one captured graph per step; with the whole set
+    as the batch every step sees the same rows, so K replays == K eager steps
+    (up to the row order inside the fp32 sums)."""
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    x = rand_inputs(256, 8)
+    y = np.stack([np.sin(x[:, 0]) + x[:, 6] / 10, np.cos(x[:, 5] / 5)], axis=1)
+    a = surrogate.Surrogate(y_mean=(0.1, 0.2), y_scale=(1.5, 0.5), batch=256, lr=2e-3, seed=4)
+    b = surrogate.Surrogate(y_mean=(0.1, 0.2), y_scale=(1.5, 0.5), batch=256, lr=2e-3, seed=4)
+    p0 = b.params()
+    a.set_data(x, y)
+    la = np.concatenate([a.train(3, seed=1), a.train(2, seed=2)])  # two calls: graph reused
+    lb = [b.step(x, y) for _ in range(5)]
+    say(f"graph losses {la}, eager {lb}")
+    assert np.allclose(la, lb, rtol=1e-4)
+    # Adam's first steps move each coordinate by ~lr * sign(g): coordinates
+    # whose gradient is ~0 may differ with the summation order, so compare norms
+    pa, pb = a.params(), b.params()
+    assert np.linalg.norm(pa - pb) <= 1e-2 * np.linalg.norm(pb - p0)
+    a.close()
+    b.close()
+
+
+def test_training_reduces_loss():
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    x = rand_inputs(8192, 9)
+    y = np.stack([np.tanh((x[:, 6] - x[:, 0] / 2) * x[:, 5] / 10) + 1, np.exp(-((x[:, 6] - 3) / 4) ** 2)], axis=1)
+    s, hist = surrogate.fit(x, y, steps=600, batch=1024, lr=2e-3, lr_final=1e-4, chunks=4, seed=0)
+    pred = s.predict(x[:2000])
+    err = np.abs(pred - y[:2000]).max(axis=0)
+    say(f"loss {hist[:5].mean():.3e} -> {hist[-20:].mean():.3e}; max |err| n {err[0]:.3e} k {err[1]:.3e}")
+    assert hist[-20:].mean() < 0.05 * hist[:5].mean()
+    assert err.max() < 0.2
+    s.close()
