@@ -77,6 +77,7 @@ SIGNATURES = {
     "ftlm_pack_result": (C.c_int, [C.c_void_p, C.c_int64, _dp, _dp, _lp]),
     "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
     "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
+    "ftlm_thermo": (C.c_int, [C.c_void_p, C.c_int64, _lp, _dp, _dp, _ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     # learned surrogate
     "ftlm_mlp_param_count": (C.c_int64, []),
     "ftlm_mlp_layout": (C.c_int, [_lp, _lp, _ip, _ip]),
@@ -343,6 +344,27 @@ class Engine:
         b = C.c_uint64(0)
         self._check(self.lib.ftlm_pack_info(self.h, C.byref(n), C.byref(b)))
         return n.value, b.value
+
+    def thermo(self, spectra, betas, mu):
+        """Device log-sum-exp sums (ftlm_thermo) of spectra = [(eig, w or
+        None for an exact block, particles, prefactor)] for every beta and mu.
+        Returns arrays (emin, z, n1, n2), each [len(betas)][len(mu)]."""
+        off = np.zeros(len(spectra) + 1, np.int64)
+        for i, (e, _, _, _) in enumerate(spectra):
+            off[i + 1] = off[i] + len(e)
+        eig = np.ascontiguousarray(np.concatenate([np.asarray(s[0], np.float64) for s in spectra])
+                                   if spectra else np.zeros(0))
+        w = np.ascontiguousarray(np.concatenate([np.ones(len(s[0])) if s[1] is None else np.asarray(s[1], np.float64)
+                                                 for s in spectra]) if spectra else np.zeros(0))
+        part = np.array([s[2] for s in spectra], np.int32)
+        pref = np.array([s[3] for s in spectra], np.float64)
+        betas = np.ascontiguousarray(betas, np.float64)
+        mu = np.ascontiguousarray(mu, np.float64)
+        out = np.zeros((betas.size, mu.size, 4))
+        self._check(self.lib.ftlm_thermo(self.h, len(spectra), _p(off, _lp), _p(eig, _dp), _p(w, _dp),
+                                         _p(part, _ip), _p(pref, _dp), betas.size, _p(betas, _dp), mu.size,
+                                         _p(mu, _dp), _p(out, _dp)))
+        return out[..., 0], out[..., 1], out[..., 2], out[..., 3]
 
     def exact_blocks(self, ks):
         """ks: list of (mx, my) of the current sector -> list of eigenvalue arrays."""

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -258,6 +259,8 @@ struct ftlm_ctx {
   DevBuf ex_mats, ex_work, ex_eig, ex_moff, ex_dims, ex_eoff, ex_views;
   // packed blocks
   PackState pack;
+  // thermodynamic sums
+  DevBuf th_e, th_w, th_n, th_beta, th_mu, th_emin, th_part, th_out;
 
   ~ftlm_ctx() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -1651,6 +1654,60 @@ int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_
     if (best_wg_per_cu) *best_wg_per_cu = bg;
     if (best_ms) *best_ms = bms;
     if (ncand) *ncand = std::min(n, cap);
+  });
+}
+
+int ftlm_thermo(ftlm_ctx* ctx, int64_t nblk, const int64_t* off, const double* eig, const double* w,
+                const int32_t* particles, const double* prefactor, int nbeta, const double* betas, int nmu,
+                const double* mu, double* out) {
+  return guarded([&] {
+    if (!ctx || !off || !particles || !prefactor || !betas || !mu || !out) throw Error(FTLM_ERR_ARG, "null argument");
+    if (nblk < 0 || nbeta <= 0 || nmu <= 0) throw Error(FTLM_ERR_ARG, "empty beta or mu list");
+    if ((int64_t)nbeta * nmu > (1 << 24)) throw Error(FTLM_ERR_LIMIT, "too many (beta, mu) pairs");
+    const int64_t nv = nblk ? off[nblk] : 0;
+    if (nv < 0 || (nv > 0 && (!eig || !w))) throw Error(FTLM_ERR_ARG, "null spectrum");
+    HIPCHECK(hipSetDevice(ctx->device));
+    // per-value weight W = prefactor * w and particle number; per-mu emin as
+    // the reference's running minimum ends (min over blocks of min E - mu N)
+    std::vector<double> W((size_t)nv);
+    std::vector<uint8_t> N((size_t)nv);
+    std::vector<double> emin((size_t)nmu, std::numeric_limits<double>::infinity());
+    for (int64_t b = 0; b < nblk; ++b) {
+      if (off[b + 1] < off[b]) throw Error(FTLM_ERR_ARG, "block offsets must be non-decreasing");
+      if (particles[b] < 0 || particles[b] > 255) throw Error(FTLM_ERR_ARG, "particle number out of range");
+      if (off[b + 1] == off[b]) continue;
+      double bmin = std::numeric_limits<double>::infinity();
+      for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+        W[(size_t)i] = prefactor[b] * w[i];
+        N[(size_t)i] = (uint8_t)particles[b];
+        bmin = std::min(bmin, eig[i]);
+      }
+      const double Nb = (double)particles[b];
+      for (int k = 0; k < nmu; ++k) emin[(size_t)k] = std::min(emin[(size_t)k], bmin - mu[k] * Nb);
+    }
+    const int np = nbeta * nmu;
+    const int64_t nwg = thermo_workgroups(nv);
+    hipStream_t s = ctx->stream;
+    double* de = ctx->th_e.as<double>((size_t)std::max<int64_t>(nv, 1));
+    double* dw = ctx->th_w.as<double>((size_t)std::max<int64_t>(nv, 1));
+    uint8_t* dn = ctx->th_n.as<uint8_t>((size_t)std::max<int64_t>(nv, 1));
+    double* db = ctx->th_beta.as<double>((size_t)nbeta);
+    double* dm = ctx->th_mu.as<double>((size_t)nmu);
+    double* dmin = ctx->th_emin.as<double>((size_t)nmu);
+    double* dp = ctx->th_part.as<double>((size_t)std::max<int64_t>(nwg, 1) * np * 3);
+    double* dout = ctx->th_out.as<double>((size_t)np * 4);
+    if (nv) {
+      HIPCHECK(hipMemcpyAsync(de, eig, (size_t)nv * 8, hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(dw, W.data(), (size_t)nv * 8, hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(dn, N.data(), (size_t)nv, hipMemcpyHostToDevice, s));
+    }
+    HIPCHECK(hipMemcpyAsync(db, betas, (size_t)nbeta * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dm, mu, (size_t)nmu * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dmin, emin.data(), (size_t)nmu * 8, hipMemcpyHostToDevice, s));
+    launch_thermo(de, dw, dn, nv, db, nbeta, dm, nmu, dmin, dp, dout, s);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(out, dout, (size_t)np * 4 * 8, hipMemcpyDeviceToHost, s));
+    sync(ctx);
   });
 }
 

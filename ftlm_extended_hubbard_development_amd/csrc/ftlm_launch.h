@@ -157,6 +157,13 @@ struct PackArgs {
 void launch_pack_reset(const PackArgs& a, int rcap, hipStream_t s);
 void launch_pack_lanczos(const PackArgs& a, int W, int grid, hipStream_t s);
 
+// ---- thermodynamic sums (ftlm_thermo.hip) ----
+// part: thermo_workgroups(nv) * nb * nm * 3 doubles; out: nb * nm * 4
+// (emin, z, n1, n2), pair p = beta index * nm + mu index.
+int64_t thermo_workgroups(int64_t nv);
+void launch_thermo(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas, int nb,
+                   const double* mus, int nm, const double* emin, double* part, double* out, hipStream_t s);
+
 // ---- exact (ftlm_exact.hip) ----
 void launch_herm_eig(const int64_t* moff, const int64_t* dims, int nblk, double* mats, double* work,
                      double* eig_out, const int64_t* eoff, hipStream_t s);

@@ -74,6 +74,41 @@ std::string twist_output(const std::string& out, double px, double py, int list_
   return out.substr(0, dot) + tag.str() + out.substr(dot);
 }
 
+// Thermodynamics of all stored spectra for every beta in one device launch
+// (ftlm_thermo), finalised as main.cpp:1770-1788.
+std::vector<Results> device_results(ftlm_ctx* c, const std::vector<Spectrum>& specs, const std::vector<double>& betas,
+                                    const std::vector<double>& mu, int sites) {
+  std::vector<std::int64_t> off(specs.size() + 1, 0);
+  for (std::size_t b = 0; b < specs.size(); ++b) off[b + 1] = off[b] + (std::int64_t)specs[b].eig.size();
+  std::vector<double> eig((std::size_t)off.back()), w((std::size_t)off.back(), 1.0), pref(specs.size());
+  std::vector<std::int32_t> part(specs.size());
+  for (std::size_t b = 0; b < specs.size(); ++b) {
+    const Spectrum& s = specs[b];
+    std::copy(s.eig.begin(), s.eig.end(), eig.begin() + off[b]);
+    if (!s.w.empty()) std::copy(s.w.begin(), s.w.end(), w.begin() + off[b]);
+    pref[b] = s.prefactor;
+    part[b] = s.particles;
+  }
+  const std::size_t nmu = mu.size();
+  std::vector<double> raw(betas.size() * nmu * 4);
+  check(ftlm_thermo(c, (std::int64_t)specs.size(), off.data(), eig.data(), w.data(), part.data(), pref.data(),
+                    (int)betas.size(), betas.data(), (int)nmu, mu.data(), raw.data()),
+        "thermodynamic sums");
+  std::vector<Results> all;
+  for (std::size_t i = 0; i < betas.size(); ++i) {
+    Thermo th(nmu);
+    for (std::size_t k = 0; k < nmu; ++k) {
+      const double* r = &raw[(i * nmu + k) * 4];
+      th.emin[k] = r[0];
+      th.z[k] = r[1];
+      th.n1[k] = r[2];
+      th.n2[k] = r[3];
+    }
+    all.push_back(finalize(th, betas[i], sites));
+  }
+  return all;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -82,6 +117,10 @@ int main(int argc, char** argv) {
     if (!parse_options(argc, argv, o, false)) return 0;
     const int sites = o.lx * o.ly;
     const bool multi = o.betas.size() > 1;
+    // tree mode: keep every spectrum and sum them on the device at the end;
+    // serial mode: the reference's streaming host accumulation (bit-identical)
+    const bool device_thermo = o.reduction == FTLM_REDUCE_TREE;
+    const bool store = multi || device_thermo;
     const std::vector<double> mu = linspace(o.mu_min, o.mu_max, o.mu_count);
     const bool trace = !o.trace_csv.empty();
     const bool dbg = o.dbg_nup >= 0 && o.dbg_ndn >= 0 && o.dbg_mx >= 0 && o.dbg_my >= 0;
@@ -175,7 +214,7 @@ int main(int argc, char** argv) {
       Spectrum& spec = it.spec;
       const int t = it.twist;
       if (it.from_ckpt) {
-        if (multi) stored[t].push_back(std::move(spec));
+        if (store) stored[t].push_back(std::move(spec));
         else
           accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
                      spec.particles, o.beta, mu, single[t], threads);
@@ -241,18 +280,12 @@ int main(int argc, char** argv) {
           wrote_dbg = true;
         }
       }
-      if (multi || !o.checkpoint.empty()) {
-        append_record(o.checkpoint, it.key, spec);
-        if (multi) {
-          stored[t].push_back(std::move(spec));
-        } else {
-          accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
-                     particles, o.beta, mu, single[t], threads);
-        }
-      } else {
-        accumulate(spec.eig.data(), use_exact ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor, particles,
-                   o.beta, mu, single[t], threads);
-      }
+      if (multi || !o.checkpoint.empty()) append_record(o.checkpoint, it.key, spec);
+      if (store)
+        stored[t].push_back(std::move(spec));
+      else
+        accumulate(spec.eig.data(), spec.w.empty() ? nullptr : spec.w.data(), spec.eig.size(), spec.prefactor,
+                   particles, o.beta, mu, single[t], threads);
     };
 
     std::deque<Item> queue;
@@ -450,27 +483,28 @@ int main(int argc, char** argv) {
       const std::string out =
           scan ? twist_output(o.output, twists[t].first, twists[t].second, o.twist_list.empty() ? -1 : t) : o.output;
       if (scan) std::cout << "twist phix=" << twists[t].first << " phiy=" << twists[t].second << "\n";
-      if (multi) {
-        std::vector<Results> all;
+      std::vector<Results> all;
+      if (device_thermo) {
+        all = device_results(ctx.c, stored[t], o.betas, mu, sites);
+      } else if (multi) {
         for (double beta : o.betas) {
           Thermo th(mu.size());
           for (const Spectrum& s : stored[t])
             accumulate(s.eig.data(), s.w.empty() ? nullptr : s.w.data(), s.eig.size(), s.prefactor, s.particles,
                        beta, mu, th, threads);
-          Results r = finalize(th, beta, sites);
-          for (std::size_t i = 0; i < mu.size(); ++i)
-            std::cout << "beta=" << std::setw(12) << beta << "  mu=" << std::setw(12) << mu[i]
-                      << "  n=" << std::setw(12) << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
-          all.push_back(std::move(r));
+          all.push_back(finalize(th, beta, sites));
         }
-        write_multi_beta(out, o.betas, mu, all);
       } else {
-        const Results r = finalize(single[t], o.beta, sites);
-        for (std::size_t i = 0; i < mu.size(); ++i)
-          std::cout << "beta=" << std::setw(12) << o.beta << "  mu=" << std::setw(12) << mu[i]
-                    << "  n=" << std::setw(12) << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
-        write_results(out, mu, r);
+        all.push_back(finalize(single[t], o.beta, sites));
       }
+      for (std::size_t b = 0; b < all.size(); ++b) {
+        const Results& r = all[b];
+        for (std::size_t i = 0; i < mu.size(); ++i)
+          std::cout << "beta=" << std::setw(12) << o.betas[b] << "  mu=" << std::setw(12) << mu[i]
+                    << "  n=" << std::setw(12) << r.n[i] << "  kappa=" << std::setw(12) << r.kappa[i] << "\n";
+      }
+      if (multi) write_multi_beta(out, o.betas, mu, all);
+      else write_results(out, mu, all[0]);
       std::cout << "Wrote " << out << "\n";
     }
     return 0;

@@ -179,6 +179,21 @@ int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_
 /* entries in the pack and device bytes they hold */
 int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes);
 
+/* Grand-canonical sums of many blocks' spectra for every (beta, mu) pair in
+ * one FP64 device launch: accumulate_block_thermo (main.cpp:1172-1219) over
+ * all blocks, with the final running minimum known up front. Block b owns
+ * values off[b]..off[b+1]-1 of eig and w (exact blocks: w = 1, prefactor 1;
+ * FTLM blocks: Ritz weights, prefactor = block dimension), particles[b] its
+ * particle number (0..255). For pair (i, k) = (betas[i], mu[k]):
+ *   emin = min over values of eig - mu * N,
+ *   z = sum prefactor * w * exp(-beta (eig - mu N - emin)), n1 = sum N * .,
+ *   n2 = sum N^2 * .;
+ * out[(i * nmu + k) * 4 + 0..3] = emin, z, n1, n2 (the reference's streaming
+ * sums up to summation-order rounding; finalise as main.cpp:1770-1788). */
+int ftlm_thermo(ftlm_ctx* ctx, int64_t nblk, const int64_t* off, const double* eig, const double* w,
+                const int32_t* particles, const double* prefactor, int nbeta, const double* betas, int nmu,
+                const double* mu, double* out);
+
 /* diagonalize_block_exact for blocks (mx[i], my[i]) of the current sector,
  * batched in one launch: eig_out receives the ascending eigenvalues of each
  * block back to back, offsets[i]..offsets[i+1]. */
