@@ -78,6 +78,10 @@ SIGNATURES = {
     "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
     "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
     "ftlm_thermo": (C.c_int, [C.c_void_p, C.c_int64, _lp, _dp, _dp, _ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ftlm_exact_queue_begin": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "ftlm_exact_queue_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _lp]),
+    "ftlm_exact_queue_run": (C.c_int, [C.c_void_p]),
+    "ftlm_exact_queue_result": (C.c_int, [C.c_void_p, C.c_int64, _dp, _lp]),
     # learned surrogate
     "ftlm_mlp_param_count": (C.c_int64, []),
     "ftlm_mlp_layout": (C.c_int, [_lp, _lp, _ip, _ip]),
@@ -344,6 +348,31 @@ class Engine:
         b = C.c_uint64(0)
         self._check(self.lib.ftlm_pack_info(self.h, C.byref(n), C.byref(b)))
         return n.value, b.value
+
+    # ---- queued exact blocks (one eigensolver launch for many sectors/twists) ----
+    def exact_queue_begin(self, budget_bytes=0):
+        self._check(self.lib.ftlm_exact_queue_begin(self.h, budget_bytes))
+        self._exq = []
+
+    def exact_queue_add(self, mx, my):
+        """Queue block (mx, my) of the current sector and twist; returns its
+        index, or None when the queue is full (nothing added)."""
+        idx = C.c_int64(-1)
+        rc = self.lib.ftlm_exact_queue_add(self.h, mx, my, C.byref(idx))
+        if rc == PACK_FULL:
+            return None
+        self._check(rc)
+        return idx.value
+
+    def exact_queue_run(self):
+        self._check(self.lib.ftlm_exact_queue_run(self.h))
+
+    def exact_queue_result(self, index):
+        cnt = C.c_int64(0)
+        self._check(self.lib.ftlm_exact_queue_result(self.h, index, None, C.byref(cnt)))
+        e = np.zeros(max(cnt.value, 1))
+        self._check(self.lib.ftlm_exact_queue_result(self.h, index, _p(e, _dp), C.byref(cnt)))
+        return e[: cnt.value]
 
     def thermo(self, spectra, betas, mu):
         """Device log-sum-exp sums (ftlm_thermo) of spectra = [(eig, w or

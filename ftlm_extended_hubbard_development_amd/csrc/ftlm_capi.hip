@@ -210,6 +210,20 @@ struct PackState {
   }
 };
 
+// Exact blocks of any sector and twist queued for one batched eigensolve:
+// each block's Hermitised dense matrix is built when it is added (its
+// sector tables are current then) into a growing device arena.
+struct ExactQueue {
+  BlockTables tab;              // tables of the block being added
+  DevBuf mats, work, eig, moff_d, dims_d, eoff_d, view_d, moff1_d;
+  size_t mats_cap = 0;          // doubles held by mats
+  uint64_t budget = 0;          // bytes of matrices per run
+  std::vector<int64_t> moff{0}, dims, eoff{0};
+  std::vector<double> eig_host;
+  int64_t solved = 0;           // entries solved by earlier runs
+  bool has_result = false;
+};
+
 }  // namespace
 
 struct ftlm_ctx {
@@ -261,6 +275,8 @@ struct ftlm_ctx {
   PackState pack;
   // thermodynamic sums
   DevBuf th_e, th_w, th_n, th_beta, th_mu, th_emin, th_part, th_out;
+  // queued exact blocks
+  ExactQueue exq;
 
   ~ftlm_ctx() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -925,6 +941,7 @@ void check_R(int R) {
 
 // ---- packed blocks ---------------------------------------------------------
 constexpr uint64_t kPackDefaultBudget = (uint64_t)64 << 30;
+constexpr uint64_t kExactDefaultBudget = (uint64_t)16 << 30;  // bytes of dense matrices per exact-queue run
 constexpr size_t kPackSlotLimit = (size_t)32 << 30;  // bytes of per-workgroup vectors
 
 size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -1544,6 +1561,117 @@ int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t*
     for (int i = 0; i <= nblk; ++i) offsets[i] = eoff[i];
     for (int64_t i = 0; i < eoff[nblk]; ++i)
       if (std::isnan(eig_out[i])) throw Error(FTLM_ERR_DEVICE, "exact eigensolver did not converge");
+  });
+}
+
+int ftlm_exact_queue_begin(ftlm_ctx* ctx, uint64_t budget_bytes) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    ExactQueue& q = ctx->exq;
+    q.budget = budget_bytes ? budget_bytes : kExactDefaultBudget;
+    q.moff.assign(1, 0);
+    q.dims.clear();
+    q.eoff.assign(1, 0);
+    q.eig_host.clear();
+    q.solved = 0;
+    q.has_result = false;
+  });
+}
+
+int ftlm_exact_queue_add(ftlm_ctx* ctx, int mx, int my, int64_t* index) {
+  bool added = true;
+  const int rc = guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    need_sector(ctx);
+    HIPCHECK(hipSetDevice(ctx->device));
+    ExactQueue& q = ctx->exq;
+    if (q.budget == 0) q.budget = kExactDefaultBudget;
+    build_block(ctx, q.tab, mx, my);
+    const int64_t n = q.tab.dim;
+    if (n <= 0) throw Error(FTLM_ERR_ARG, "empty block");
+    if (n > 8192) throw Error(FTLM_ERR_LIMIT, "exact diagonalisation is limited to dim <= 8192");
+    const int64_t o = q.moff.back();
+    const size_t need = 2 * (size_t)(o + n * n);  // doubles
+    if (!q.dims.empty() && need * 8 > q.budget) {
+      added = false;
+      return;
+    }
+    hipStream_t s = ctx->stream;
+    if (need > q.mats_cap) {  // grow, keeping the matrices built so far
+      const size_t cap = std::max(need, 2 * q.mats_cap);
+      DevBuf grown;
+      double* p = grown.as<double>(cap);
+      if (o > 0) HIPCHECK(hipMemcpyAsync(p, q.mats.ptr<double>(), 16 * (size_t)o, hipMemcpyDeviceToDevice, s));
+      sync(ctx);
+      std::swap(q.mats.p, grown.p);
+      std::swap(q.mats.bytes, grown.bytes);
+      q.mats_cap = cap;
+    }
+    double* A = q.mats.ptr<double>() + 2 * o;
+    launch_zero(A, 2 * n * n, s);
+    const int64_t mo[2] = {0, n * n};
+    int64_t* dmo = q.moff1_d.as<int64_t>(2);
+    BlockView* dv = q.view_d.as<BlockView>(1);
+    HIPCHECK(hipMemcpyAsync(dmo, mo, sizeof(mo), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(dv, &q.tab.view, sizeof(BlockView), hipMemcpyHostToDevice, s));
+    launch_dense_build(gdev(ctx), dv, 1, dmo, A, n, s);
+    HIPCHECK(hipGetLastError());
+    sync(ctx);  // the staging above and the block tables are reused by the next add
+    q.dims.push_back(n);
+    q.moff.push_back(o + n * n);
+    q.eoff.push_back(q.eoff.back() + n);
+    q.has_result = false;
+    if (index) *index = (int64_t)q.dims.size() - 1;
+  });
+  if (rc != FTLM_OK) return rc;
+  return added ? FTLM_OK : FTLM_PACK_FULL;
+}
+
+int ftlm_exact_queue_run(ftlm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ExactQueue& q = ctx->exq;
+    // entries added since the last run (the solver overwrites its matrices,
+    // so solved entries keep their eigenvalues and are not solved again)
+    const int64_t nb = (int64_t)q.dims.size();
+    const int64_t first = q.solved;
+    q.eig_host.resize((size_t)q.eoff.back(), 0.0);
+    if (nb > first) {
+      if (nb - first > INT32_MAX) throw Error(FTLM_ERR_LIMIT, "too many queued blocks");
+      hipStream_t s = ctx->stream;
+      int64_t* dmoff = q.moff_d.as<int64_t>(nb + 1);
+      int64_t* ddims = q.dims_d.as<int64_t>(nb);
+      int64_t* deoff = q.eoff_d.as<int64_t>(nb + 1);
+      double* work = q.work.as<double>(6 * (size_t)q.eoff.back() + 6);
+      double* eig = q.eig.as<double>((size_t)q.eoff.back() + 1);
+      HIPCHECK(hipMemcpyAsync(dmoff, q.moff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(ddims, q.dims.data(), nb * 8, hipMemcpyHostToDevice, s));
+      HIPCHECK(hipMemcpyAsync(deoff, q.eoff.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+      launch_herm_eig(dmoff + first, ddims + first, (int)(nb - first), q.mats.ptr<double>(), work, eig,
+                      deoff + first, s);
+      HIPCHECK(hipGetLastError());
+      const int64_t e0 = q.eoff[(size_t)first];
+      HIPCHECK(hipMemcpyAsync(q.eig_host.data() + e0, eig + e0, (size_t)(q.eoff.back() - e0) * 8,
+                              hipMemcpyDeviceToHost, s));
+      sync(ctx);
+      for (size_t i = (size_t)e0; i < q.eig_host.size(); ++i)
+        if (std::isnan(q.eig_host[i])) throw Error(FTLM_ERR_DEVICE, "exact eigensolver did not converge");
+      q.solved = nb;
+    }
+    q.has_result = true;
+  });
+}
+
+int ftlm_exact_queue_result(ftlm_ctx* ctx, int64_t index, double* eig, int64_t* count) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    const ExactQueue& q = ctx->exq;
+    if (index < 0 || index >= (int64_t)q.dims.size()) throw Error(FTLM_ERR_ARG, "queue index out of range");
+    if (index >= q.solved) throw Error(FTLM_ERR_STATE, "exact queue entry has not been run");
+    const int64_t a = q.eoff[(size_t)index], b = q.eoff[(size_t)index + 1];
+    if (eig) std::copy(q.eig_host.begin() + a, q.eig_host.begin() + b, eig);
+    if (count) *count = b - a;
   });
 }
 

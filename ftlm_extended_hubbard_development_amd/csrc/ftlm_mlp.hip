@@ -5,7 +5,7 @@
 // mu) to the per-site density n and the compressibility kappa that the
 // engine's FTLM sweeps produce:
 //
-//   12 encoded features (zero padded to 32)
+//   16 encoded features (zero padded to 32)
 //     -> 4 hidden layers of 256 units, GELU (erf form)
 //     -> 2 outputs (padded to 32), in units of a per-output normalisation.
 //
@@ -400,7 +400,7 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 void encode_row(const double* x, float* f) {
   const double U = x[0], V = x[1], tp = x[2], px = x[3], py = x[4], beta = x[5], mu = x[6];
-  double e[12];
+  double e[16];
   e[0] = (U - 7.0) / 3.0;
   e[1] = V - 1.0;
   e[2] = tp / 0.3;
@@ -413,7 +413,13 @@ void encode_row(const double* x, float* f) {
   e[9] = (mu - 5.0) / 10.0;
   e[10] = (mu - 0.5 * U) / 5.0;
   e[11] = beta * (mu - 0.5 * U) / 100.0;
-  for (int i = 0; i < kFeat; ++i) f[i] = i < 12 ? (float)e[i] : 0.0f;
+  // beta (mu - U/2 - 4V): distance to the half-filling point of the
+  // four-neighbour model in units of temperature
+  e[12] = beta * mu / 100.0;
+  e[13] = beta * (mu - 0.5 * U - 4.0 * V) / 100.0;
+  e[14] = (mu - 0.5 * U - 4.0 * V) / 5.0;
+  e[15] = beta * U / 100.0;
+  for (int i = 0; i < kFeat; ++i) f[i] = i < 16 ? (float)e[i] : 0.0f;
 }
 
 }  // namespace

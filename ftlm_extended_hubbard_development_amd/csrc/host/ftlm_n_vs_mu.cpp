@@ -58,6 +58,9 @@ struct Item {
   Spectrum spec;
   std::vector<double> exact;  // exact eigenvalues (exact, trace or debug blocks)
   std::int64_t pack_idx = -1;
+  std::int64_t exact_idx = -1;
+  bool need_pack = false;   // waits for its pack's run
+  bool need_exact = false;  // waits for the exact queue's run
   bool ready = false;
 };
 
@@ -331,7 +334,8 @@ int main(int argc, char** argv) {
         check(ftlm_pack_result(ctx.c, it.pack_idx, it.spec.eig.data(), it.spec.w.data(), &cnt), "packed FTLM");
         it.spec.eig.resize((std::size_t)cnt);
         it.spec.w.resize((std::size_t)cnt);
-        it.ready = true;
+        it.need_pack = false;
+        it.ready = !it.need_exact;
       }
       ftlm_stats st{};
       ftlm_last_stats(ctx.c, &st);
@@ -342,6 +346,31 @@ int main(int argc, char** argv) {
       pending.clear();
       pack_entries = 0;
       check(ftlm_pack_begin(ctx.c, o.samples, o.steps, budget), "pack");
+      drain();
+    };
+
+    // ---- the queue of exact blocks (one batched eigensolve) ----
+    std::vector<std::size_t> ex_pending;  // push positions waiting for the queue
+    std::int64_t exact_entries = 0;
+    check(ftlm_exact_queue_begin(ctx.c, 0), "exact diagonalization");
+    auto flush_exact = [&]() {
+      if (exact_entries == 0) return;
+      const auto t0 = std::chrono::steady_clock::now();
+      check(ftlm_exact_queue_run(ctx.c), "exact diagonalization");
+      for (std::size_t pos : ex_pending) {
+        Item& it = queue[pos - popped];
+        std::int64_t cnt = 0;
+        check(ftlm_exact_queue_result(ctx.c, it.exact_idx, nullptr, &cnt), "exact diagonalization");
+        it.exact.resize((std::size_t)cnt);
+        check(ftlm_exact_queue_result(ctx.c, it.exact_idx, it.exact.data(), &cnt), "exact diagonalization");
+        it.need_exact = false;
+        it.ready = !it.need_pack;
+      }
+      tj_rec("\"method\":\"exact_batch\",\"blocks\":" + std::to_string(exact_entries) +
+             ",\"ms\":" + std::to_string(ms_since(t0)));
+      ex_pending.clear();
+      exact_entries = 0;
+      check(ftlm_exact_queue_begin(ctx.c, 0), "exact diagonalization");
       drain();
     };
 
@@ -357,40 +386,10 @@ int main(int argc, char** argv) {
               << ", which exceeds --max-sector-dim=" << o.max_sector_dim;
           die(msg.str());
         }
-        // block dimensions (twist independent) and the exact batch per twist
+        // block dimensions (twist independent)
         std::vector<std::int64_t> dims(L);
-        std::vector<int> exm, exy;
-        std::vector<int> exidx(L, -1);
         for (int my = 0; my < o.ly; ++my)
-          for (int mx = 0; mx < o.lx; ++mx) {
-            const int k = my * o.lx + mx;
-            check(ftlm_block(ctx.c, mx, my, &dims[k]), "block build");
-            if (dims[k] == 0) continue;
-            if (ckpt.count(BlockKey{nup, ndn, mx, my})) continue;
-            const bool use_exact = (std::size_t)dims[k] <= o.threshold;
-            const bool dbg_here = dbg && nup == o.dbg_nup && ndn == o.dbg_ndn && mx == o.dbg_mx && my == o.dbg_my;
-            if (use_exact || trace || dbg_here) {
-              exidx[k] = (int)exm.size();
-              exm.push_back(mx);
-              exy.push_back(my);
-            }
-          }
-        std::vector<std::vector<double>> exeig(ntw);
-        std::vector<std::int64_t> exoff(exm.size() + 1, 0);
-        if (!exm.empty()) {
-          std::int64_t tot = 0;
-          for (std::size_t i = 0; i < exm.size(); ++i) tot += dims[exy[i] * o.lx + exm[i]];
-          for (int t = 0; t < ntw; ++t) {
-            use_twist(t);
-            exeig[t].resize((std::size_t)tot);
-            const auto t0 = std::chrono::steady_clock::now();
-            check(ftlm_exact_blocks(ctx.c, (int)exm.size(), exm.data(), exy.data(), exeig[t].data(), exoff.data()),
-                  "exact diagonalization");
-            tj_rec("\"nup\":" + std::to_string(nup) + ",\"ndn\":" + std::to_string(ndn) +
-                   ",\"method\":\"exact_batch\",\"blocks\":" + std::to_string(exm.size()) +
-                   ",\"ms\":" + std::to_string(ms_since(t0)));
-          }
-        }
+          for (int mx = 0; mx < o.lx; ++mx) check(ftlm_block(ctx.c, mx, my, &dims[my * o.lx + mx]), "block build");
         std::size_t active = 0;
         for (int my = 0; my < o.ly; ++my) {
           for (int mx = 0; mx < o.lx; ++mx) {
@@ -420,15 +419,24 @@ int main(int argc, char** argv) {
               it.spec.particles = it.particles;
               it.spec.dim = (std::uint64_t)dim;
               it.spec.exact = it.use_exact;
-              if (exidx[k] >= 0)
-                it.exact.assign(exeig[t].begin() + exoff[exidx[k]], exeig[t].begin() + exoff[exidx[k] + 1]);
+              use_twist(t);
+              // exact eigenvalues (exact blocks, and trace/debug references)
+              // come from the queued batch eigensolve
+              it.need_exact = it.use_exact || trace || it.dbg_here;
+              if (it.need_exact) {
+                int rc = ftlm_exact_queue_add(ctx.c, mx, my, &it.exact_idx);
+                if (rc == FTLM_PACK_FULL) {
+                  flush_exact();
+                  rc = ftlm_exact_queue_add(ctx.c, mx, my, &it.exact_idx);
+                }
+                check(rc == FTLM_PACK_FULL ? FTLM_ERR_LIMIT : rc, "exact diagonalization");
+                ++exact_entries;
+              }
               if (it.use_exact) {
-                it.ready = true;
-                push(std::move(it));
+                ex_pending.push_back(push(std::move(it)));
                 continue;
               }
               it.spec.prefactor = (double)dim;
-              use_twist(t);
               check(ftlm_block(ctx.c, mx, my, nullptr), "block build");
               const std::uint64_t bseed = ftlm_block_seed(o.seed, nup, ndn, mx, my);
               if (packing && dim <= o.pack_max_dim) {
@@ -439,7 +447,11 @@ int main(int argc, char** argv) {
                 }
                 check(rc == FTLM_PACK_FULL ? FTLM_ERR_LIMIT : rc, "pack");
                 ++pack_entries;
-                pending.push_back(push(std::move(it)));
+                it.need_pack = true;
+                const bool ex = it.need_exact;
+                const std::size_t pos = push(std::move(it));
+                pending.push_back(pos);
+                if (ex) ex_pending.push_back(pos);
                 continue;
               }
               const auto t0 = std::chrono::steady_clock::now();
@@ -452,14 +464,22 @@ int main(int argc, char** argv) {
                     "FTLM block");
               it.spec.eig.resize((std::size_t)cnt);
               it.spec.w.resize((std::size_t)cnt);
-              it.ready = true;
+              it.ready = !it.need_exact;
               tj_rec("\"nup\":" + std::to_string(nup) + ",\"ndn\":" + std::to_string(ndn) + ",\"mx\":" +
                      std::to_string(mx) + ",\"my\":" + std::to_string(my) + ",\"dim\":" + std::to_string(dim) +
                      ",\"method\":\"ftlm\",\"ms\":" + std::to_string(ms_since(t0)));
-              push(std::move(it));
+              const bool ex = it.need_exact;
+              const std::size_t pos = push(std::move(it));
+              if (ex) ex_pending.push_back(pos);
             }
             drain();
           }
+        }
+        // a checkpointed run writes every sector's records before the next
+        // sector starts, so an interrupted run keeps its progress
+        if (!o.checkpoint.empty()) {
+          flush_pack();
+          flush_exact();
         }
         std::ostringstream line;
         line << "  sector Nup=" << nup << " Ndown=" << ndn << " full_dim=" << full << " parents=" << parents
@@ -473,6 +493,7 @@ int main(int argc, char** argv) {
       }
     }
     flush_pack();
+    flush_exact();
     drain();
     if (!queue.empty()) die("internal error: unfinished blocks");
     if (tj) tj << "\n]\n";

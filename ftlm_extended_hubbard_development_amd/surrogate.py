@@ -62,7 +62,7 @@ def param_count():
 
 # inputs actually used by the first layer and outputs of the last one; the
 # rest of the padded widths stays zero
-USED_FEATURES = 12
+USED_FEATURES = 16
 USED_OUTPUTS = 2
 
 
@@ -199,16 +199,15 @@ class Surrogate:
                    device=device)
 
 
-def fit(x, y, steps=6000, batch=4096, lr=2e-3, lr_final=2e-5, chunks=12, seed=0, device=0, log=None):
-    """Train a new surrogate on (x, y): targets normalised by their mean and
-    standard deviation, Adam with a geometric learning-rate decay from lr to
-    lr_final over `chunks` equal parts. Returns (surrogate, loss history)."""
-    x = np.asarray(x, np.float64).reshape(-1, 7)
-    y = np.asarray(y, np.float64).reshape(-1, 2)
-    mean = y.mean(axis=0)
-    scale = np.maximum(y.std(axis=0), 1e-6)
-    s = Surrogate(y_mean=tuple(mean), y_scale=tuple(scale), batch=batch, lr=lr, seed=seed, device=device)
-    s.set_data(x, y)
+# the compressibility is scaled by this many standard deviations more than n
+# in the loss: its peaks (kappa = beta * charge fluctuation) would otherwise
+# dominate, while the FTLM gate is on n
+KAPPA_DOWNWEIGHT = 4.0
+
+
+def train_schedule(s, steps, lr, lr_final, chunks, seed, log=None):
+    """Adam with a geometric learning-rate decay from lr to lr_final over
+    `chunks` equal parts; returns the loss history."""
     hist = []
     per = max(1, steps // chunks)
     done = 0
@@ -222,7 +221,29 @@ def fit(x, y, steps=6000, batch=4096, lr=2e-3, lr_final=2e-5, chunks=12, seed=0,
         done += n
         if log:
             log(f"chunk {c}: lr {rate:.2e} loss {float(hist[-1][-min(50, n):].mean()):.3e}")
-    return s, np.concatenate(hist)
+    return np.concatenate(hist)
+
+
+def fit(x, y, steps=6000, batch=4096, lr=2e-3, lr_final=2e-5, chunks=12, seed=0, device=0, log=None):
+    """Train a new surrogate on (x, y): targets normalised by their mean and
+    standard deviation (the compressibility's scaled KAPPA_DOWNWEIGHT times
+    more), Adam with a geometric learning-rate decay. Returns (surrogate,
+    loss history)."""
+    x = np.asarray(x, np.float64).reshape(-1, 7)
+    y = np.asarray(y, np.float64).reshape(-1, 2)
+    mean = y.mean(axis=0)
+    scale = np.maximum(y.std(axis=0), 1e-6) * np.array([1.0, KAPPA_DOWNWEIGHT])
+    s = Surrogate(y_mean=tuple(mean), y_scale=tuple(scale), batch=batch, lr=lr, seed=seed, device=device)
+    s.set_data(x, y)
+    return s, train_schedule(s, steps, lr, lr_final, chunks, seed, log)
+
+
+def finetune(base, x, y, steps=2000, batch=1024, lr=5e-4, lr_final=1e-5, chunks=4, seed=0, device=0):
+    """A copy of `base` (same normalisation) trained further on (x, y)."""
+    s = Surrogate(y_mean=base.y_mean, y_scale=base.y_scale, batch=batch, lr=lr, params=base.params(),
+                  device=device)
+    s.set_data(x, y)
+    return s, train_schedule(s, steps, lr, lr_final, chunks, seed)
 
 
 # ---------------------------------------------------------------- FTLM data
@@ -241,9 +262,24 @@ def _read_multi_beta_csv(path):
 
 
 def ftlm_twists(lattice, u, v, tp, twists, betas, mu_min=-5.0, mu_max=15.0, mu_count=81, samples=32, steps=80,
-                seed=12345, exe=FTLM_EXE, workdir=None, extra=()):
+                seed=12345, exe=FTLM_EXE, chunk=48, log=None, extra=()):
     """Run the drop-in FTLM executable (GPU) for one model and a list of
-    twists in one process. Returns (rows (n, 7) inputs, (n, 2) outputs, wall s)."""
+    twists, `chunk` twists per process. Returns (rows (n, 7) inputs, (n, 2)
+    outputs, wall seconds)."""
+    xs, ys, wall = [], [], 0.0
+    for c0 in range(0, len(twists), chunk):
+        x, y, w = _ftlm_twists_once(lattice, u, v, tp, twists[c0:c0 + chunk], betas, mu_min, mu_max, mu_count,
+                                    samples, steps, seed, exe, extra)
+        xs.append(x)
+        ys.append(y)
+        wall += w
+        if log:
+            log(f"  FTLM {lattice} twists {c0}..{min(len(twists), c0 + chunk) - 1} of {len(twists)}: {w:.2f}s")
+    return np.concatenate(xs), np.concatenate(ys), wall
+
+
+def _ftlm_twists_once(lattice, u, v, tp, twists, betas, mu_min, mu_max, mu_count, samples, steps, seed, exe,
+                      extra, workdir=None):
     lx, ly = parse_lattice(lattice)
     betas = [float(b) for b in betas]
     nb = len(betas)
@@ -314,59 +350,121 @@ def twist_grid(n):
 
 
 def scan(sur, lattice, u, v, tp, grid, betas, mu_min=-5.0, mu_max=15.0, mu_count=81, judge_stride=4, tol=TOL_N,
-         samples=32, steps=80, seed=12345):
+         samples=32, steps=80, seed=12345, finetune_steps=2000, log=None):
     """Dense twist scan (grid x grid twists, all betas and mu) filled by the
-    surrogate. FTLM runs on the judge twists (every `judge_stride`-th twist in
-    each direction); a twist keeps the surrogate's values only when every
-    judge of its cell (the corners of the judge sub-grid cell holding it,
-    periodic) agrees with FTLM within `tol` in n at every (beta, mu);
-    otherwise FTLM is run for it (fallback). Judge twists take FTLM's values.
+    surrogate, with FTLM as the judge and the fallback.
+
+    The twist grid is cut into cells of judge_stride x judge_stride twists.
+    FTLM runs on two coarse sub-grids: the cell corners ("fit" twists) and the
+    cell centres ("check" twists). With finetune_steps > 0 a copy of the
+    surrogate is first trained further on the fit twists' FTLM rows (the
+    scan's own model parameters); the check twists are never trained on. A
+    cell keeps the surrogate's values only when its check twist agrees with
+    FTLM within `tol` in n at every (beta, mu); every other twist of a failing
+    cell is computed by FTLM. Fit and check twists always take FTLM's values.
 
     Returns dict: x (rows of inputs), y (served values), source (0 surrogate,
-    1 FTLM), judge_err (per judge twist), timings."""
-    if grid % judge_stride:
-        raise ValueError("grid must be a multiple of judge_stride")
+    1 FTLM) per row and per twist, check errors, timings."""
+    s = judge_stride
+    if grid % s or s < 2:
+        raise ValueError("grid must be a multiple of judge_stride >= 2")
     tw = twist_grid(grid)
     mu = np.linspace(mu_min, mu_max, mu_count)
     betas = [float(b) for b in betas]
     nb, nm = len(betas), mu_count
-    # surrogate everywhere
+    ftlm = dict(mu_min=mu_min, mu_max=mu_max, mu_count=mu_count, samples=samples, steps=steps, seed=seed, log=log)
+    fit_idx = [iy * grid + ix for iy in range(0, grid, s) for ix in range(0, grid, s)]
+    chk_idx = [iy * grid + ix for iy in range(s // 2, grid, s) for ix in range(s // 2, grid, s)]
+    judges = fit_idx + chk_idx
+    xj, yj, t_judge = ftlm_twists(lattice, u, v, tp, [tw[i] for i in judges], betas, **ftlm)
+    yj = yj.reshape(len(judges), nb * nm, 2)
     t0 = time.perf_counter()
+    model = sur
+    if finetune_steps > 0:
+        rows = len(fit_idx) * nb * nm
+        model, _ = finetune(sur, xj[:rows], yj[:len(fit_idx)].reshape(-1, 2), steps=finetune_steps,
+                            batch=min(1024, (rows // 64) * 64 or 64))
     xs = np.array([[u, v, tp, px, py, b, m] for (px, py) in tw for b in betas for m in mu])
-    ys = sur.predict(xs).reshape(len(tw), nb * nm, 2)
+    ys = model.predict(xs).reshape(len(tw), nb * nm, 2)
+    if model is not sur:
+        model.close()
     t_sur = time.perf_counter() - t0
-    # judges
-    jidx = [iy * grid + ix for iy in range(0, grid, judge_stride) for ix in range(0, grid, judge_stride)]
-    xj, yj, t_judge = ftlm_twists(lattice, u, v, tp, [tw[i] for i in jidx], betas, mu_min, mu_max, mu_count,
-                                  samples, steps, seed)
-    yj = yj.reshape(len(jidx), nb * nm, 2)
-    jerr = np.abs(ys[jidx, :, 0] - yj[:, :, 0]).max(axis=1)
-    jbad = {i: e > tol for i, e in zip(jidx, jerr)}
+    chk_err = np.abs(ys[chk_idx, :, 0] - yj[len(fit_idx):, :, 0]).max(axis=1)
+    cell_ok = {}
+    for i, e in zip(chk_idx, chk_err):
+        cell_ok[((i % grid) // s, (i // grid) // s)] = e <= tol
     src = np.zeros(len(tw), np.int8)
     out = ys.copy()
-    for k, i in enumerate(jidx):
+    for k, i in enumerate(judges):
         out[i] = yj[k]
         src[i] = 1
-    fall = []
-    for i, (px, py) in enumerate(tw):
-        if src[i]:
-            continue
-        ix, iy = i % grid, i // grid
-        x0, y0 = ix - ix % judge_stride, iy - iy % judge_stride
-        corners = [((y0 + dy) % grid) * grid + (x0 + dx) % grid for dy in (0, judge_stride) for dx in (0, judge_stride)]
-        if any(jbad[c] for c in corners):
-            fall.append(i)
+    fall = [i for i in range(len(tw)) if not src[i] and not cell_ok[((i % grid) // s, (i // grid) // s)]]
     t_fall = 0.0
     if fall:
-        _, yf, t_fall = ftlm_twists(lattice, u, v, tp, [tw[i] for i in fall], betas, mu_min, mu_max, mu_count,
-                                    samples, steps, seed)
+        _, yf, t_fall = ftlm_twists(lattice, u, v, tp, [tw[i] for i in fall], betas, **ftlm)
         yf = yf.reshape(len(fall), nb * nm, 2)
         for k, i in enumerate(fall):
             out[i] = yf[k]
             src[i] = 1
-    return {"x": xs, "y": out.reshape(-1, 2), "source": np.repeat(src, nb * nm), "twist_source": src,
-            "judges": jidx, "judge_err": jerr, "fallback": fall, "t_surrogate": t_sur, "t_judge": t_judge,
-            "t_fallback": t_fall}
+    return {"x": xs, "y": out.reshape(-1, 2), "y_surrogate": ys.reshape(-1, 2), "source": np.repeat(src, nb * nm),
+            "twist_source": src, "fit": fit_idx, "check": chk_idx, "check_err": chk_err, "fallback": fall,
+            "t_surrogate": t_sur, "t_judge": t_judge, "t_fallback": t_fall}
+
+
+def evaluate(sur, x, y, tol=TOL_N):
+    """Error statistics of the surrogate on held-out rows: |dn| quantiles,
+    the fraction within `tol`, the maximum per beta, and |dkappa|."""
+    pred = sur.predict(x)
+    dn = np.abs(pred[:, 0] - y[:, 0])
+    dk = np.abs(pred[:, 1] - y[:, 1])
+    per_beta = {f"{b:.4g}": float(dn[x[:, 5] == b].max()) for b in np.unique(x[:, 5])}
+    return {"rows": int(x.shape[0]), "dn_max": float(dn.max()), "dn_p50": float(np.quantile(dn, 0.5)),
+            "dn_p90": float(np.quantile(dn, 0.9)), "dn_p99": float(np.quantile(dn, 0.99)),
+            "frac_within_tol": float((dn <= tol).mean()), "dkappa_max": float(dk.max()),
+            "dkappa_p50": float(np.quantile(dk, 0.5)), "dn_max_per_beta": per_beta}
+
+
+def gain(sur, lattice, u, v, tp, grid, betas, judge_stride=4, tol=TOL_N, samples=32, steps=80, finetune_steps=2000,
+         mu_min=-5.0, mu_max=15.0, mu_count=81, log=None):
+    """What the surrogate gains on a dense twist scan: the judged scan
+    (surrogate + FTLM judges + FTLM fallback) against FTLM on every twist
+    (the method without the surrogate). Returns times, the number of twists
+    the surrogate served and the error of the served values against the
+    all-FTLM scan (judge values are FTLM's own, so only surrogate-served rows
+    can differ)."""
+    mus = dict(mu_min=mu_min, mu_max=mu_max, mu_count=mu_count)
+    r = scan(sur, lattice, u, v, tp, grid, betas, judge_stride=judge_stride, tol=tol, samples=samples, steps=steps,
+             finetune_steps=finetune_steps, log=log, **mus)
+    tw = twist_grid(grid)
+    _, yf, t_full = ftlm_twists(lattice, u, v, tp, tw, betas, samples=samples, steps=steps, log=log, **mus)
+    dn = np.abs(r["y"][:, 0] - yf[:, 0])
+    sur_rows = r["source"] == 0
+    t_scan = r["t_surrogate"] + r["t_judge"] + r["t_fallback"]
+    # the same scan at other tolerances, from the measured pieces: cells
+    # accepted at `t`, fallback time at the measured FTLM time per twist, the
+    # error of what the surrogate then serves against FTLM (the all-FTLM run)
+    nb_nm = yf.shape[0] // len(tw)
+    t_twist = t_full / len(tw)
+    ypred = r["y_surrogate"].reshape(len(tw), nb_nm, 2)
+    dn_sur = np.abs(ypred[:, :, 0] - yf.reshape(len(tw), nb_nm, 2)[:, :, 0]).max(axis=1)
+    judged = set(r["fit"]) | set(r["check"])
+    s = judge_stride
+    curve = []
+    for t in sorted({tol, 2e-3, 5e-3, 1e-2, 2e-2, 5e-2}):
+        ok = {((i % grid) // s, (i // grid) // s) for i, e in zip(r["check"], r["check_err"]) if e <= t}
+        served = [i for i in range(len(tw)) if i not in judged and ((i % grid) // s, (i // grid) // s) in ok]
+        n_fall = len(tw) - len(judged) - len(served)
+        t_est = r["t_surrogate"] + r["t_judge"] + n_fall * t_twist
+        curve.append({"tol": t, "surrogate_twists": len(served), "speedup": t_full / t_est,
+                      "dn_max_served": float(max((dn_sur[i] for i in served), default=0.0))})
+    return {"twists": len(tw), "judges": len(judged), "fallback": len(r["fallback"]),
+            "surrogate_twists": int((r["twist_source"] == 0).sum()),
+            "t_ftlm_all": t_full, "t_scan": t_scan, "t_surrogate": r["t_surrogate"], "t_judge": r["t_judge"],
+            "t_fallback": r["t_fallback"], "speedup": t_full / t_scan,
+            "dn_max_served": float(dn.max()),
+            "dn_max_surrogate_rows": float(dn[sur_rows].max()) if sur_rows.any() else 0.0,
+            "dn_p50_surrogate_rows": float(np.quantile(dn[sur_rows], 0.5)) if sur_rows.any() else 0.0,
+            "check_err": [float(e) for e in r["check_err"]], "tolerance_curve": curve}
 
 
 # ---------------------------------------------------------------------- CLI
@@ -389,19 +487,25 @@ def _main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", required=True)
-    s = sub.add_parser("scan")
-    s.add_argument("model")
-    s.add_argument("--lattice", required=True)
-    s.add_argument("--u", type=float, required=True)
-    s.add_argument("--v", type=float, default=0.0)
-    s.add_argument("--tp", type=float, default=0.0)
-    s.add_argument("--grid", type=int, default=16)
-    s.add_argument("--judge-stride", type=int, default=4)
-    s.add_argument("--betas", default="2,5,10")
-    s.add_argument("--tol", type=float, default=TOL_N)
-    s.add_argument("--samples", type=int, default=32)
-    s.add_argument("--steps", type=int, default=80)
-    s.add_argument("--out", required=True)
+    e = sub.add_parser("evaluate")
+    e.add_argument("model")
+    e.add_argument("data", nargs="+")
+    for name in ("scan", "gain"):
+        s = sub.add_parser(name)
+        s.add_argument("model")
+        s.add_argument("--lattice", required=True)
+        s.add_argument("--u", type=float, required=True)
+        s.add_argument("--v", type=float, default=0.0)
+        s.add_argument("--tp", type=float, default=0.0)
+        s.add_argument("--grid", type=int, default=16)
+        s.add_argument("--judge-stride", type=int, default=4)
+        s.add_argument("--betas", default="2,5,10")
+        s.add_argument("--tol", type=float, default=TOL_N)
+        s.add_argument("--samples", type=int, default=32)
+        s.add_argument("--steps", type=int, default=80)
+        s.add_argument("--finetune-steps", type=int, default=2000)
+        if name == "scan":
+            s.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
     def log(m):
@@ -410,7 +514,7 @@ def _main(argv=None):
     if a.cmd == "generate":
         x, y, secs = generate(a.lattice, a.draws, a.twists, mu_count=a.mu_count, samples=a.samples, steps=a.steps,
                               seed=a.seed, log=log)
-        np.savez(a.out, x=x, y=y, ftlm_seconds=secs, lattice=a.lattice)
+        np.savez_compressed(a.out, x=x, y=y, ftlm_seconds=secs, lattice=a.lattice)
         log(f"wrote {a.out}: {x.shape[0]} rows, {secs:.1f}s of FTLM")
     elif a.cmd == "train":
         xs, ys = [], []
@@ -423,14 +527,24 @@ def _main(argv=None):
         sur, hist = fit(x, y, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed, log=log)
         sur.save(a.out, meta={"data": a.data, "steps": a.steps, "train_s": time.perf_counter() - t0})
         log(f"wrote {a.out}: final loss {float(hist[-50:].mean()):.3e}")
+    elif a.cmd == "evaluate":
+        sur = Surrogate.load(a.model)
+        for p in a.data:
+            d = np.load(p, allow_pickle=False)
+            log(json.dumps({"data": p, **evaluate(sur, d["x"], d["y"])}))
     else:
         sur = Surrogate.load(a.model)
-        r = scan(sur, a.lattice, a.u, a.v, a.tp, a.grid, [float(b) for b in a.betas.split(",")],
-                 judge_stride=a.judge_stride, tol=a.tol, samples=a.samples, steps=a.steps)
-        np.savez(a.out, **{k: np.asarray(v) for k, v in r.items()})
-        log(json.dumps({"twists": a.grid ** 2, "judges": len(r["judges"]), "fallback": len(r["fallback"]),
-                        "surrogate_twists": int((r["twist_source"] == 0).sum()),
-                        "t_surrogate": r["t_surrogate"], "t_judge": r["t_judge"], "t_fallback": r["t_fallback"]}))
+        kw = dict(judge_stride=a.judge_stride, tol=a.tol, samples=a.samples, steps=a.steps,
+                  finetune_steps=a.finetune_steps, log=log)
+        betas = [float(b) for b in a.betas.split(",")]
+        if a.cmd == "gain":
+            log(json.dumps(gain(sur, a.lattice, a.u, a.v, a.tp, a.grid, betas, **kw)))
+        else:
+            r = scan(sur, a.lattice, a.u, a.v, a.tp, a.grid, betas, **kw)
+            np.savez(a.out, **{k: np.asarray(v) for k, v in r.items()})
+            log(json.dumps({"twists": a.grid ** 2, "judges": len(r["fit"]) + len(r["check"]),
+                            "fallback": len(r["fallback"]), "surrogate_twists": int((r["twist_source"] == 0).sum()),
+                            "t_surrogate": r["t_surrogate"], "t_judge": r["t_judge"], "t_fallback": r["t_fallback"]}))
     return 0
 
 

@@ -179,6 +179,26 @@ int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_
 /* entries in the pack and device bytes they hold */
 int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes);
 
+/* diagonalize_block_exact (main.cpp:847-911) deferred and batched across
+ * sectors and twists: the reference diagonalises each exact block where the
+ * block loop meets it; here each block's Hermitised dense matrix is built on
+ * the device when it is added (from the current sector and twist) and one
+ * launch solves every queued block.
+ *   ftlm_exact_queue_begin   empties the queue; budget = bytes of matrices
+ *                            (0: 16 GiB)
+ *   ftlm_exact_queue_add     queues block (mx, my) of the current sector at
+ *                            the current twist; *index its position; returns
+ *                            FTLM_PACK_FULL, adding nothing, when it does not
+ *                            fit the budget (run the queue first)
+ *   ftlm_exact_queue_run     solves every block queued since the last run
+ *                            (earlier results stay available)
+ *   ftlm_exact_queue_result  ascending eigenvalues of entry `index` (*count
+ *                            of them) */
+int ftlm_exact_queue_begin(ftlm_ctx* ctx, uint64_t budget_bytes);
+int ftlm_exact_queue_add(ftlm_ctx* ctx, int mx, int my, int64_t* index);
+int ftlm_exact_queue_run(ftlm_ctx* ctx);
+int ftlm_exact_queue_result(ftlm_ctx* ctx, int64_t index, double* eig, int64_t* count);
+
 /* Grand-canonical sums of many blocks' spectra for every (beta, mu) pair in
  * one FP64 device launch: accumulate_block_thermo (main.cpp:1172-1219) over
  * all blocks, with the final running minimum known up front. Block b owns
@@ -205,7 +225,7 @@ int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t*
  * two observables the reference's CSV reports per mu (main.cpp:1409-1455),
  * so that dense parameter and twist scans can be filled between FTLM runs;
  * FTLM stays the judge and the fallback. Inputs are rows of 7 doubles in that
- * order, outputs rows of 2 doubles. The network: 12 encoded features (zero
+ * order, outputs rows of 2 doubles. The network: 16 encoded features (zero
  * padded to FTLM_MLP_FEATURES) -> 4 hidden layers of 256 GELU units -> 2
  * outputs (padded to 32), output c = y_mean[c] + y_scale[c] * raw output.
  * Weights are bf16 copies of fp32 master parameters; every product runs on

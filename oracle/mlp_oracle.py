@@ -5,7 +5,7 @@ Loaded by tests/ as the checker of the GPU surrogate kernels
 product imports it. The surrogate has no counterpart in the reference, so
 this file restates the network the engine implements, in float64 numpy:
 
-  features X (n x 32; 12 used) -> 4 x [Z = H W^T + b, H = GELU(Z)] (256 wide)
+  features X (n x 32; 16 used) -> 4 x [Z = H W^T + b, H = GELU(Z)] (256 wide)
   -> Y = H W_4^T + b_4 (32 wide; 2 used),
   loss L = sum_{rows < n, c < 2} (Y - T)^2 / (2 n),
 
@@ -70,6 +70,10 @@ def encode(x):
     f[:, 9] = (mu - 5.0) / 10.0
     f[:, 10] = (mu - 0.5 * U) / 5.0
     f[:, 11] = beta * (mu - 0.5 * U) / 100.0
+    f[:, 12] = beta * mu / 100.0
+    f[:, 13] = beta * (mu - 0.5 * U - 4.0 * V) / 100.0
+    f[:, 14] = (mu - 0.5 * U - 4.0 * V) / 5.0
+    f[:, 15] = beta * U / 100.0
     return f.astype(np.float32)
 
 

@@ -109,7 +109,7 @@ def test_gradients_match_oracle(sur, n):
             assert r_emu <= 5e-3, (l, name, r_emu)
             assert r_ex <= 5e-2, (l, name, r_ex)
     # padding receives no gradient
-    assert not g[mo.W_OFF[0]:mo.B_OFF[0]].reshape(256, 32)[:, 12:].any()
+    assert not g[mo.W_OFF[0]:mo.B_OFF[0]].reshape(256, 32)[:, 16:].any()
     assert not g[mo.W_OFF[4]:mo.B_OFF[4]].reshape(32, 256)[2:, :].any()
 
 

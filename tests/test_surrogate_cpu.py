@@ -77,13 +77,13 @@ def test_encoding_matches_engine():
     ref = mo.encode(x)
     assert ours.dtype == np.float32 and ours.shape == (4000, 32)
     assert np.abs(ours - ref).max() <= 1e-7
-    assert not ours[:, 12:].any()
+    assert not ours[:, 16:].any()
 
 
 def test_init_params_padding_zero():
     p = rand_params(5)
     layers = mo.unpack(p)
-    assert not layers[0][0][:, 12:].any()
+    assert not layers[0][0][:, 16:].any()
     assert not layers[4][0][2:, :].any() and not layers[4][1][2:].any()
     assert np.abs(layers[1][0]).max() <= 1 / 16 + 1e-12
 
@@ -118,7 +118,7 @@ def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(7)
     idx = []
     for l, (n, k) in enumerate(mo.SHAPES):  # coordinates of every weight and bias tensor (used part)
-        kk = 12 if l == 0 else k
+        kk = 16 if l == 0 else k
         nn = 2 if l == 4 else n
         idx += [mo.W_OFF[l] + int(rng.integers(nn)) * k + int(rng.integers(kk)) for _ in range(4)]
         idx += [mo.B_OFF[l] + int(rng.integers(nn)) for _ in range(2)]
