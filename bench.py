@@ -68,6 +68,145 @@ def cpu_baseline(budget_s=25.0):
                        f"run_ftlm_block incl. Jacobi, {dt:.1f} s on {threads} threads")}
 
 
+def _interp_log(xs, ys, x):
+    """Piecewise-linear interpolation in log(x), clamped at the ends."""
+    import numpy as np
+
+    return float(np.interp(np.log(x), np.log(np.asarray(xs, float)), np.asarray(ys, float)))
+
+
+def sweep_estimate(ftlm, R=128, m=80, pack_every=8, log=None):
+    """Seconds of the whole 4x4 U=7 sweep (the CLI's block loop over all 4,564
+    (N_up, N_down, k) blocks, R vectors, m Lanczos steps), batched and plain
+    per-vector, from measured parts:
+
+    * basis: every sector and block built (measured, whole sweep);
+    * exact blocks (dim <= 256): all 708 in one queued eigensolve (measured);
+    * packed FTLM blocks (256 < dim <= 131072): every `pack_every`-th of them
+      in one persistent launch, extrapolated by algorithmic bytes;
+    * solo FTLM blocks (dim > 131072): a stratified sample by dimension
+      (quantiles of the solo range, the largest block included), each run
+      batched; seconds per byte interpolated in log(dim) over all solo blocks;
+    * plain per-vector mode: one vector of sampled blocks over the whole
+      dimension range with the R = 1 kernels, seconds per vector interpolated
+      in log(dim), times R per block.
+    Device time of the Lanczos passes; the start vectors are drawn on the GPU
+    (the CLI's default with tree reductions), their time reported separately."""
+    import numpy as np
+
+    say = log or (lambda s: None)
+    L, lx, ly = 16, 4, 4
+    eng = ftlm.Engine(lx=lx, ly=ly, u=7.0)
+    eng.set_start_source(ftlm.START_DEVICE)  # the CLI's default with tree reductions
+    t0 = time.perf_counter()
+    blocks = []
+    for nup in range(L + 1):
+        for ndn in range(L + 1):
+            eng.sector(nup, ndn)
+            for my in range(ly):
+                for mx in range(lx):
+                    d = eng.block(mx, my)
+                    if d:
+                        blocks.append((nup, ndn, mx, my, d))
+    t_basis = time.perf_counter() - t0
+    say(f"# sweep: {len(blocks)} blocks enumerated in {t_basis:.2f}s")
+
+    def algo_bytes(b):
+        nup, ndn, _, _, d = b
+        k = min(m, d)
+        return k * d * R * 16.0 * (hbar(L, nup, ndn, 4) + 2.0) + k * 4.0 * d * R * 16.0  # apply + recurrence
+
+    exact = [b for b in blocks if b[4] <= 256]
+    packed = [b for b in blocks if 256 < b[4] <= 131072]
+    solo = [b for b in blocks if b[4] > 131072]
+
+    # exact blocks: all, one queued eigensolve
+    t0 = time.perf_counter()
+    eng.exact_queue_begin()
+    cur = None
+    for b in exact:
+        if cur != b[:2]:
+            eng.sector(*b[:2])
+            cur = b[:2]
+        eng.exact_queue_add(b[2], b[3])
+    eng.exact_queue_run()
+    t_exact = time.perf_counter() - t0
+    say(f"# sweep: {len(exact)} exact blocks {t_exact:.2f}s")
+
+    # packed blocks: a deterministic 1/pack_every sample in one pack
+    sample = packed[::pack_every]
+    t0 = time.perf_counter()
+    eng.pack_begin(R, m)
+    cur = None
+    for b in sample:
+        if cur != b[:2]:
+            eng.sector(*b[:2])
+            cur = b[:2]
+        eng.block(b[2], b[3])
+        if eng.pack_add(ftlm.block_seed(12345, *b[:4])) is None:
+            raise RuntimeError("pack sample does not fit one pack")
+    t_fill = time.perf_counter() - t0
+    eng.pack_tune(3)
+    st = eng.pack_run(0, 0, 1)
+    pack_s = st["ms_total"] / 1000.0
+    pack_rate = sum(algo_bytes(b) for b in sample) / pack_s
+    t_packed = sum(algo_bytes(b) for b in packed) / pack_rate
+    say(f"# sweep: pack of {len(sample)}/{len(packed)} blocks {pack_s:.3f}s device, fill {t_fill:.1f}s "
+        f"-> {t_packed:.1f}s for all packed blocks")
+
+    # solo blocks and plain per-vector: stratified samples
+    dsolo = sorted(b[4] for b in solo)
+    want = [dsolo[int(q * (len(dsolo) - 1))] for q in (0.0, 0.25, 0.5, 0.75, 0.9)] + [dsolo[-1]]
+    pick = []
+    for w in want:
+        b = min(solo, key=lambda x: (abs(x[4] - w), x))
+        if b not in pick:
+            pick.append(b)
+    dpk = sorted(b[4] for b in packed)
+    plain_pick = [min(packed, key=lambda x: (abs(x[4] - dpk[int(q * (len(dpk) - 1))]), x))
+                  for q in (0.0, 0.33, 0.66, 1.0)]
+    solo_rate, plain_vec = [], []
+    t_host_starts = 0.0
+    for b in plain_pick + pick:
+        eng.sector(*b[:2])
+        d = eng.block(b[2], b[3])
+        a = time.perf_counter()
+        eng.prepare(R, ftlm.block_seed(12345, *b[:4]))
+        t_host_starts += time.perf_counter() - a
+        if b in pick:
+            s = eng.run(R, m, ftlm.REDUCE_TREE)
+            a = time.perf_counter()
+            eng.finish(R, m)
+            fin = time.perf_counter() - a
+            secs = s["ms_total"] / 1000.0 + fin
+            solo_rate.append((d, algo_bytes(b) / secs))
+            say(f"# sweep: solo block {b[:4]} dim {d}: {secs:.3f}s ({algo_bytes(b) / secs / 1e12:.2f} TB/s)")
+        a = time.perf_counter()
+        eng.run(R, m, ftlm.REDUCE_TREE, per_vector_column=0)
+        pv = time.perf_counter() - a
+        plain_vec.append((d, pv))
+        say(f"# sweep: plain vector {b[:4]} dim {d}: {pv * 1000:.2f} ms")
+    eng.close()
+    solo_rate.sort()
+    plain_vec.sort()
+    t_solo = sum(algo_bytes(b) / _interp_log([x[0] for x in solo_rate], [x[1] for x in solo_rate], b[4])
+                 for b in solo)
+    t_plain_ftlm = sum(R * _interp_log([x[0] for x in plain_vec], [x[1] for x in plain_vec], b[4])
+                       for b in packed + solo)
+    batched = t_basis + t_exact + t_packed + t_solo
+    plain = t_basis + t_exact + t_plain_ftlm
+    return {"metric": "4x4 sweep sec (extrapolated from a stratified sample)", "batched_s": round(batched, 1),
+            "plain_s": round(plain, 1), "batched_vs_plain": round(plain / batched, 2),
+            "parts_s": {"basis": round(t_basis, 2), "exact": round(t_exact, 2), "packed": round(t_packed, 1),
+                        "solo": round(t_solo, 1), "plain_ftlm": round(t_plain_ftlm, 1)},
+            "blocks": {"exact": len(exact), "packed": len(packed), "solo": len(solo),
+                       "packed_sampled": len(sample), "solo_sampled": len(pick),
+                       "plain_sampled": len(plain_pick) + len(pick)},
+            "solo_rate_TBps": [(d, round(r / 1e12, 3)) for d, r in solo_rate],
+            "plain_ms_per_vector": [(d, round(t * 1000, 2)) for d, t in plain_vec],
+            "start_vectors_s_sampled": round(t_host_starts, 2)}
+
+
 def read_traffic():
     """HBM bytes per H-apply launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -88,6 +227,8 @@ def main():
     ap.add_argument("--block", default="8,8,0,0")
     ap.add_argument("--plain-vectors", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the whole-sweep estimate (batched vs plain, from a stratified block sample)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
@@ -166,6 +307,12 @@ def main():
     speedup = (plain_per_vec * R / ms_per_step) if plain_per_vec else None
     eng.close()
 
+    sweep = None
+    if not args.no_sweep:
+        try:
+            sweep = sweep_estimate(ftlm, R, m, log=lambda t: print(t, file=sys.stderr, flush=True))
+        except Exception as e:  # reported, not hidden: the per-block measurement above stands
+            sweep = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu else cpu_baseline()
     hb = hbar(16, nup, ndn, 4)
     per_launch = dim * R * 16.0 * (hb + 2.0)
@@ -194,6 +341,7 @@ def main():
         "plain_per_vector_ms": round(plain_per_vec, 2) if plain_per_vec else None,
         "batched_vs_plain": round(speedup, 2) if speedup else None,
         "cpu_baseline": cpu,
+        "sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 

@@ -23,6 +23,8 @@ ED_EXE = os.path.join(PKG_DIR, "ftlm_ed_n_vs_mu")
 
 REDUCE_SERIAL = 0
 REDUCE_TREE = 1
+START_HOST = 0
+START_DEVICE = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -78,6 +80,7 @@ SIGNATURES = {
     "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
     "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
     "ftlm_thermo": (C.c_int, [C.c_void_p, C.c_int64, _lp, _dp, _dp, _ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ftlm_set_start_source": (C.c_int, [C.c_void_p, C.c_int]),
     "ftlm_exact_queue_begin": (C.c_int, [C.c_void_p, C.c_uint64]),
     "ftlm_exact_queue_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _lp]),
     "ftlm_exact_queue_run": (C.c_int, [C.c_void_p]),
@@ -348,6 +351,11 @@ class Engine:
         b = C.c_uint64(0)
         self._check(self.lib.ftlm_pack_info(self.h, C.byref(n), C.byref(b)))
         return n.value, b.value
+
+    def set_start_source(self, source):
+        """START_HOST (libstdc++ stream on host threads, bit-identical to the
+        reference) or START_DEVICE (the same stream drawn on the GPU)."""
+        self._check(self.lib.ftlm_set_start_source(self.h, source))
 
     # ---- queued exact blocks (one eigensolver launch for many sectors/twists) ----
     def exact_queue_begin(self, budget_bytes=0):

@@ -277,6 +277,9 @@ struct ftlm_ctx {
   DevBuf th_e, th_w, th_n, th_beta, th_mu, th_emin, th_part, th_out;
   // queued exact blocks
   ExactQueue exq;
+  // where seeded start vectors are drawn: FTLM_START_HOST or FTLM_START_DEVICE
+  int start_source = FTLM_START_HOST;
+  DevBuf start_norms, start_one;
 
   ~ftlm_ctx() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -624,6 +627,15 @@ void ensure_vectors(ftlm_ctx* c, int R) {
 void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout) {
   const int64_t dim = c->cur.dim;
   ensure_vectors(c, R);
+  if (c->start_source == FTLM_START_DEVICE) {
+    launch_start_vectors(bseed, 0, 0, R, R, dim, layout, c->vec[0].ptr<double>(), c->start_norms.as<double>(R),
+                         c->stream);
+    HIPCHECK(hipGetLastError());
+    sync(c);
+    c->prepared_R = R;
+    c->prepared_layout = layout;
+    return;
+  }
   // pinned staging of at most ~2 GB (or one sample)
   const int64_t fit = std::max<int64_t>(1, (int64_t)(2ll << 30) / std::max<int64_t>(1, dim * 16));
   const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)R, (int64_t)host_threads(), fit}));
@@ -1172,6 +1184,14 @@ bool pack_add(ftlm_ctx* c, uint64_t bseed, int64_t* index) {
   HIPCHECK(hipMemcpyAsync(ptab, b.view.ptab, 128 * (size_t)L, hipMemcpyDeviceToDevice, s));
   launch_hops_fill(gdev(c), b.view, rp, srp, col, code, spec, diag, s);
   HIPCHECK(hipGetLastError());
+  if (!start && c->start_source == FTLM_START_DEVICE) {
+    double2* st = static_cast<double2*>(p.arena.alloc((size_t)16 * n * p.R));
+    launch_start_vectors(bseed, 0, 0, p.R, p.R, n, 1, reinterpret_cast<double*>(st), c->start_norms.as<double>(p.R),
+                         s);
+    HIPCHECK(hipGetLastError());
+    start = st;
+    p.starts.push_back({{bseed, n}, st});
+  }
   if (!start) {
     double2* st = static_cast<double2*>(p.arena.alloc((size_t)16 * n * p.R));
     const int64_t fit = std::max<int64_t>(1, (int64_t)(1ll << 30) / std::max<int64_t>(1, n * 16));
@@ -1368,7 +1388,30 @@ int ftlm_start_vector(ftlm_ctx* ctx, uint64_t block_seed, int sample, double* ou
   return guarded([&] {
     if (!ctx || !out) throw Error(FTLM_ERR_ARG, "null argument");
     need_block(ctx);
+    if (sample < 0) throw Error(FTLM_ERR_ARG, "sample must be non-negative");
+    if (ctx->start_source == FTLM_START_DEVICE) {
+      HIPCHECK(hipSetDevice(ctx->device));
+      const int64_t dim = ctx->cur.dim;
+      double* d = ctx->start_one.as<double>(2 * (size_t)dim);
+      launch_start_vectors(block_seed, sample, 0, 1, 1, dim, 1, d, ctx->start_norms.as<double>(1), ctx->stream);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(out, d, 16 * (size_t)dim, hipMemcpyDeviceToHost, ctx->stream));
+      sync(ctx);
+      return;
+    }
     gen_start(block_seed, sample, ctx->cur.dim, out);
+  });
+}
+
+int ftlm_set_start_source(ftlm_ctx* ctx, int source) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    if (source != FTLM_START_HOST && source != FTLM_START_DEVICE) throw Error(FTLM_ERR_ARG, "unknown start source");
+    if (source != ctx->start_source) {
+      ctx->start_source = source;
+      ctx->prepared_R = 0;  // prepared start vectors came from the other source
+      ctx->pack.starts.clear();
+    }
   });
 }
 

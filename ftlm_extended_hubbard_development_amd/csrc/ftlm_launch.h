@@ -157,6 +157,13 @@ struct PackArgs {
 void launch_pack_reset(const PackArgs& a, int rcap, hipStream_t s);
 void launch_pack_lanczos(const PackArgs& a, int W, int grid, hipStream_t s);
 
+// ---- seeded start vectors on the device (ftlm_rng.hip) ----
+// Samples r0..r0+cnt-1 of block seed bseed, dim rows each, normalised, into
+// output columns col0..col0+cnt-1 of out: layout 0 [dim][R] (column c),
+// layout 1 [R][dim] (row c); norms: cnt doubles of scratch.
+void launch_start_vectors(uint64_t bseed, int r0, int col0, int cnt, int R, int64_t dim, int layout, double* out,
+                          double* norms, hipStream_t s);
+
 // ---- thermodynamic sums (ftlm_thermo.hip) ----
 // part: thermo_workgroups(nv) * nb * nm * 3 doubles; out: nb * nm * 4
 // (emin, z, n1, n2), pair p = beta index * nm + mu index.

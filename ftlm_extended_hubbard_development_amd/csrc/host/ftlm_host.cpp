@@ -95,6 +95,8 @@ void print_help(const char* argv0) {
             << "  --reduction MODE      Lanczos reductions: tree (default) or serial\n"
             << "                        (serial sums in row order, bit-identical to the CPU code)\n"
             << "  --per-vector          run each random vector alone (plain per-vector GPU FTLM)\n"
+            << "  --host-starts         draw start vectors on host threads with tree reductions too\n"
+            << "                        (tree reductions draw the same seeded stream on the GPU)\n"
             << "  --timing-json PATH    write per-block device timings\n"
             << "  --pack-max-dim N      FTLM blocks up to N rows run packed in persistent launches\n"
             << "                        (default 131072; 0 runs every block on its own)\n"
@@ -153,6 +155,7 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
       else if (m == "serial") p.reduction = 0;
       else die("--reduction must be tree or serial");
     } else if (arg == "--per-vector") p.per_vector = true;
+    else if (arg == "--host-starts") p.host_starts = true;
     else if (arg == "--timing-json") p.timing_json = next(arg);
     else if (arg == "--pack-max-dim") p.pack_max_dim = parse_as<std::int64_t>(next(arg));
     else if (arg == "--pack-width") p.pack_width = parse_as<int>(next(arg));

@@ -38,6 +38,9 @@ struct Options {
   double pack_budget_gb = 0.0;      // device bytes per pack (0: engine default)
   // twist scan: phix, phiy in {0, 1/N, ..., (N-1)/N}, one CSV per twist
   int twist_grid = 0;
+  // start vectors drawn on host threads even with tree reductions (which
+  // otherwise draw the same stream on the GPU)
+  bool host_starts = false;
   // twist scan over listed (phix, phiy) pairs, one CSV per twist
   std::vector<std::pair<double, double>> twist_list;
 };

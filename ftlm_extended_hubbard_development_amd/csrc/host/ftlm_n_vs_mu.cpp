@@ -170,6 +170,9 @@ int main(int argc, char** argv) {
     Ctx ctx;
     const ftlm_model model{o.lx, o.ly, o.tx, o.ty, o.tp, twists[0].first, twists[0].second, o.u, o.v};
     check(ftlm_create(&model, o.device, &ctx.c), "GPU engine unavailable");
+    // serial reductions reproduce the reference bit for bit: host-drawn starts
+    check(ftlm_set_start_source(ctx.c, device_thermo && !o.host_starts ? FTLM_START_DEVICE : FTLM_START_HOST),
+          "start vectors");
     int cur_tw = 0;
     auto use_twist = [&](int t) {
       if (t == cur_tw) return;

@@ -92,8 +92,22 @@ int ftlm_block_csr(ftlm_ctx* ctx, int all_dirs, int64_t* nnz, int32_t* row_ptr, 
 int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y);
 
 /* Seeded unit-norm complex Gaussian start vector of one sample
- * (main.cpp:1015-1025), 2*dim doubles. */
+ * (main.cpp:1015-1025), 2*dim doubles, from the context's start source. */
 int ftlm_start_vector(ftlm_ctx* ctx, uint64_t block_seed, int sample, double* out);
+
+/* Where the seeded start vectors of every later Lanczos run, FTLM block and
+ * pack entry of the context are drawn (no reference counterpart; the stream
+ * is the reference's in both cases, main.cpp:1015-1025):
+ *   FTLM_START_HOST    libstdc++ mt19937_64 + normal_distribution on host
+ *                      threads, bit-identical to the reference (default);
+ *   FTLM_START_DEVICE  the same engine and polar method on the GPU, written
+ *                      straight into device memory; elements differ from the
+ *                      host stream only in the last bit where the device log
+ *                      rounds differently from glibc's, and the normalisation
+ *                      is summed in tree order. */
+#define FTLM_START_HOST 0
+#define FTLM_START_DEVICE 1
+int ftlm_set_start_source(ftlm_ctx* ctx, int source);
 
 /* Lanczos coefficients of R vectors on the current block.
  * start: host [dim][R] complex, or NULL for the seeded vectors of samples
