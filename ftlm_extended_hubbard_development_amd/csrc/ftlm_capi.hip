@@ -73,6 +73,11 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
   void* get(size_t n) {
     if (n == 0) n = 8;
     if (n > bytes) {
@@ -198,6 +203,8 @@ struct PackState {
   std::vector<int32_t> m;
   // tuned launch shape
   int best_w = 0, best_wg = 0;
+  // reductions of the next launches: 1 tree (QL runs), 0 row order (Jacobi runs)
+  int tree_red = 1;
   DevBuf d_entries, d_items, d_nitems, queue, fail, alpha, beta, mcount, eig_d, w_d, scratch, iscratch, slots;
   PinnedBuf stage;
   hipGraphExec_t exec = nullptr;
@@ -954,7 +961,7 @@ void check_R(int R) {
 // ---- packed blocks ---------------------------------------------------------
 constexpr uint64_t kPackDefaultBudget = (uint64_t)64 << 30;
 constexpr uint64_t kExactDefaultBudget = (uint64_t)16 << 30;  // bytes of dense matrices per exact-queue run
-constexpr size_t kPackSlotLimit = (size_t)32 << 30;  // bytes of per-workgroup vectors
+constexpr size_t kPackSlotLimit = (size_t)32 << 30;  // bytes of per-workgroup vectors kept between runs
 
 size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -977,6 +984,17 @@ std::vector<PackItem> pack_items(const PackState& p, int W) {
 }
 
 // Launch state for one run: uploads the tables and returns the kernel args.
+// Device bytes the per-workgroup slots may take in one launch: half of what
+// is free (counting what the slots hold already), between 32 and 128 GiB.
+// Wide column groups of the largest packed blocks need this to fill the GPU;
+// ftlm_pack_run gives anything above 32 GiB back after the run.
+size_t pack_slot_limit(ftlm_ctx* c) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return kPackSlotLimit;
+  const size_t avail = free_b + c->pack.slots.bytes;
+  return std::min(std::max(kPackSlotLimit, avail / 2), (size_t)128 << 30);
+}
+
 PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax) {
   PackState& p = c->pack;
   const std::vector<PackItem> items = pack_items(p, W);
@@ -985,7 +1003,7 @@ PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax) {
   p.rcap = rcap;
   const int64_t stride = std::max<int64_t>(1, p.max_dim) * W;
   const size_t per_wg = (size_t)3 * stride * 16;
-  int g = std::max(1, std::min(*grid, (int)std::min<size_t>(kPackSlotLimit / per_wg, (size_t)1 << 20)));
+  int g = std::max(1, std::min(*grid, (int)std::min<size_t>(pack_slot_limit(c) / per_wg, (size_t)1 << 20)));
   g = std::min<int64_t>(g, (int64_t)items.size());
   *grid = g;
   PackArgs a{};
@@ -1006,6 +1024,7 @@ PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax) {
   a.slot_stride = stride;
   a.rcap = rcap;
   a.kmax = kmax;
+  a.tree = p.tree_red;
   a.alpha = p.alpha.as<double>((size_t)p.steps * rcap);
   a.beta = p.beta.as<double>((size_t)p.steps * rcap);
   a.mcount = p.mcount.as<int32_t>(rcap);
@@ -1037,6 +1056,7 @@ double pack_run(ftlm_ctx* c, int width, int wg_per_cu, int method) {
   if (!p.open) throw Error(FTLM_ERR_STATE, "no pack (call ftlm_pack_begin first)");
   if (method != 0 && method != 1) throw Error(FTLM_ERR_ARG, "method must be 0 (Jacobi) or 1 (QL)");
   p.has_result = false;
+  p.tree_red = method == 1 ? 1 : 0;
   const int W = pack_width(p, width);
   if (p.entries.empty()) {
     p.has_result = true;
@@ -1770,6 +1790,10 @@ int ftlm_pack_run(ftlm_ctx* ctx, int width, int wg_per_cu, int method) {
     if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
     HIPCHECK(hipSetDevice(ctx->device));
     (void)pack_run(ctx, width, wg_per_cu, method);
+    if (ctx->pack.slots.bytes > kPackSlotLimit) {  // leave the memory to solo blocks
+      sync(ctx);
+      ctx->pack.slots.release();
+    }
   });
 }
 
@@ -1804,7 +1828,7 @@ int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_
     double bms = 0.0;
     for (int W : {8, 16, 32, 64}) {
       if (W > 16 && W >= 2 * p.R) continue;
-      for (int wg : {1, 2, 4}) {
+      for (int wg : {1, 2}) {  // 4 per CU measured no better (throughput-bound)
         const double ms = pack_time(ctx, W, wg, tune_steps > 0 ? tune_steps : 4);
         if (n < cap) {
           if (cand_width) cand_width[n] = W;

@@ -150,6 +150,9 @@ struct PackArgs {
   int64_t slot_stride;        // >= max dim * W
   int32_t rcap;               // column capacity (row stride of alpha/beta)
   int32_t kmax;               // > 0: stop after kmax steps (tuning runs)
+  int32_t tree;               // 0: column sums in row order (bit-identical to the
+                              // reference); 1: fused per-thread partials reduced
+                              // in a fixed order (tree reductions)
   double* alpha;              // [steps][rcap]
   double* beta;               // [steps][rcap]
   int32_t* mcount;            // [rcap]

@@ -80,17 +80,26 @@ __device__ __forceinline__ double col_norm2(const double2* __restrict__ x, int64
   return s;
 }
 
+// Reductions: with a.tree == 0 one thread per column sums its rows in row
+// order (bit-identical to the reference). With a.tree == 1 the alpha dot is
+// accumulated inside the apply loop and the norm inside the update loop,
+// per thread (thread p of a column owns rows p, p + NP, ...), and the NP
+// partials of a column are added in index order: two vector passes fewer
+// per step and no serial phase; a fixed order, so runs repeat exactly.
 template <int W>
-__global__ void __launch_bounds__(kPackThreads) k_pack_lanczos(PackArgs a) {
-  constexpr int RPW = 64 / W;  // rows per wave pass
+__global__ void __launch_bounds__(kPackThreads, 4) k_pack_lanczos(PackArgs a) {
+  constexpr int RPW = 64 / W;             // rows per wave pass
+  constexpr int NP = kPackWaves * RPW;    // threads per column
   __shared__ int s_item;
   __shared__ double s_alpha[W], s_b[W], s_bprev[W];
+  __shared__ double s_red[NP][W];
   __shared__ int s_done[W];
   __shared__ int s_live;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int c = lane & (W - 1);
   const int rs = lane / W;
+  const bool tree = a.tree != 0;
   double2* x0 = a.slots + (int64_t)blockIdx.x * 3 * a.slot_stride;
   double2* x1 = x0 + a.slot_stride;
   double2* y = x1 + a.slot_stride;
@@ -98,7 +107,7 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_lanczos(PackArgs a) {
   for (;;) {
     if (tid == 0) s_item = (int)atomicAdd(a.queue, 1u);
     __syncthreads();
-    const int it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item and entry fields load into scalar registers
     __syncthreads();
     if (it >= n_items) break;
     const PackItem item = a.items[it];
@@ -121,10 +130,12 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_lanczos(PackArgs a) {
     }
     __syncthreads();
     const int kdim = a.kmax > 0 && a.kmax < e.kdim ? a.kmax : e.kdim;
+    const int p = wave * RPW + rs;  // this thread's rows: p, p + NP, ... (column c)
     for (int k = 0; k < kdim; ++k) {
       // y = H x0 (main.cpp:833-845: diagonal first, then hops in order)
       const bool act = !s_done[c];
-      for (int64_t i = (int64_t)wave * RPW + rs; i < dim; i += (int64_t)kPackWaves * RPW) {
+      double dpart = 0.0;
+      for (int64_t i = p; i < dim; i += NP) {
         if (!act) continue;
         const double dg = e.diag[i];
         const double2 xs = x0[i * W + c];
@@ -150,24 +161,45 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_lanczos(PackArgs a) {
           acc = cadd(acc, cmul(pack_coef(e.code[h], e.ptab, e.spec), {t.x, t.y}));
         }
         y[i * W + c] = make_double2(acc.re, acc.im);
+        if (tree) dpart = dpart + (xs.x * acc.re + xs.y * acc.im);
+      }
+      if (tree) s_red[p][c] = dpart;
+      __syncthreads();
+      // alpha = Re <x0, y> (main.cpp:1037): serial over rows, or the partials
+      if (tid < W && !s_done[tid]) {
+        if (tree) {
+          double s = 0.0;
+          for (int q = 0; q < NP; ++q) s = s + s_red[q][tid];
+          s_alpha[tid] = s;
+        } else {
+          s_alpha[tid] = col_dot(x0, y, dim, W, tid);
+        }
       }
       __syncthreads();
-      // alpha = Re <x0, y>, serial over rows (main.cpp:1037)
-      if (tid < W && !s_done[tid]) s_alpha[tid] = col_dot(x0, y, dim, W, tid);
-      __syncthreads();
-      // x1 = (y - alpha x0) - bprev x1 (main.cpp:1039-1041)
+      // x1 = (y - alpha x0) - bprev x1 (main.cpp:1039-1041); thread (p, c)
+      // again owns rows p, p + NP, ...
+      double npart = 0.0;
       for (int64_t t = tid; t < dim * W; t += kPackThreads) {
         const int cc = (int)(t & (W - 1));
         if (s_done[cc]) continue;
         const double al = s_alpha[cc], b = s_bprev[cc];
         const double2 hw = y[t], hq = x0[t], hp = x1[t];
-        x1[t] = make_double2((hw.x - al * hq.x) - b * hp.x, (hw.y - al * hq.y) - b * hp.y);
+        const double2 v = make_double2((hw.x - al * hq.x) - b * hp.x, (hw.y - al * hq.y) - b * hp.y);
+        x1[t] = v;
+        if (tree) npart = npart + (v.x * v.x + v.y * v.y);
       }
+      if (tree) s_red[p][c] = npart;
       __syncthreads();
       // beta = sqrt(|x1|^2) and the stopping rule (main.cpp:1043-1056)
       if (tid < W && !s_done[tid]) {
         const int64_t gc = col + tid;
-        const double nrm = col_norm2(x1, dim, W, tid);
+        double nrm;
+        if (tree) {
+          nrm = 0.0;
+          for (int q = 0; q < NP; ++q) nrm = nrm + s_red[q][tid];
+        } else {
+          nrm = col_norm2(x1, dim, W, tid);
+        }
         a.alpha[(int64_t)k * a.rcap + gc] = s_alpha[tid];
         a.mcount[gc] = k + 1;
         const double b = sqrt(nrm);

@@ -3,15 +3,19 @@
 // through the C ABI of include/ftlm.h.
 //
 // Same flags, stdout lines, CSV columns, checkpoint/.meta files and block
-// order (N_up, N_down, my, mx) as the reference. Per sector, the exact
-// blocks are diagonalised in one batched launch. FTLM blocks up to
-// --pack-max-dim rows are added to a pack that runs as one persistent launch
-// once full (or at the end); larger blocks run on their own. Blocks are
-// finished (trace/debug output, checkpoint record, accumulation) strictly in
-// the reference's order from an in-order queue, so every output keeps the
-// reference's order whatever ran when. --twist-grid N visits every block for
-// all N*N twists, packing the twists of a block together (their start
-// vectors are shared: block seeds do not depend on the twist).
+// order (N_up, N_down, my, mx) as the reference. Exact blocks of every
+// sector and twist are queued and diagonalised in one batched launch. FTLM
+// blocks up to --pack-max-dim rows are added to a pack that runs as one
+// persistent launch once full (or at the end); larger blocks run on their
+// own. Queue and pack also run at the end of every sector when
+// checkpointing. Blocks are finished (trace/debug output, checkpoint record,
+// accumulation) strictly in the reference's order from an in-order queue, so
+// every output keeps the reference's order whatever ran when. With tree
+// reductions the start vectors are drawn on the GPU and the thermodynamic
+// sums of all blocks run on the GPU at the end. --twist-grid N /
+// --twist-list visit every block for all twists, packing the twists of a
+// block together (their start vectors are shared: block seeds do not depend
+// on the twist).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -202,7 +206,7 @@ int main(int argc, char** argv) {
     bool first_tj = true;
     auto tj_rec = [&](const std::string& body) {
       if (!tj) return;
-      tj << (first_tj ? "" : ",\n") << "{" << body << "}";
+      tj << (first_tj ? "" : ",\n") << "{" << body << "}" << std::flush;
       first_tj = false;
     };
     const int L = sites;
@@ -323,7 +327,9 @@ int main(int argc, char** argv) {
       if (!tuned) {
         int bw = 0, bg = 0, n = 0;
         double bms = 0.0;
-        check(ftlm_pack_tune(ctx.c, 3, &bw, &bg, &bms, &n, nullptr, nullptr, nullptr), "pack tuning");
+        // two Lanczos steps per candidate (the first one also loads the start
+        // vectors): enough to rank (width, grid), a small share of the run
+        check(ftlm_pack_tune(ctx.c, 2, &bw, &bg, &bms, &n, nullptr, nullptr, nullptr), "pack tuning");
         tuned = true;
         tj_rec("\"method\":\"pack_tune\",\"width\":" + std::to_string(bw) + ",\"wg_per_cu\":" + std::to_string(bg));
       }
