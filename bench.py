@@ -75,7 +75,7 @@ def _interp_log(xs, ys, x):
     return float(np.interp(np.log(x), np.log(np.asarray(xs, float)), np.asarray(ys, float)))
 
 
-def sweep_estimate(ftlm, R=128, m=80, pack_every=8, log=None):
+def sweep_estimate(ftlm, R=128, m=80, pack_every=8, validate=0, log=None):
     """Seconds of the whole 4x4 U=7 sweep (the CLI's block loop over all 4,564
     (N_up, N_down, k) blocks, R vectors, m Lanczos steps), batched and plain
     per-vector, from measured parts:
@@ -86,7 +86,8 @@ def sweep_estimate(ftlm, R=128, m=80, pack_every=8, log=None):
       in one persistent launch, extrapolated by algorithmic bytes;
     * solo FTLM blocks (dim > 131072): a stratified sample by dimension
       (quantiles of the solo range, the largest block included), each run
-      batched; seconds per byte interpolated in log(dim) over all solo blocks;
+      batched; seconds per byte interpolated in log(dim) over all solo blocks
+      (--sweep-validate N times N other solo blocks against that prediction);
     * plain per-vector mode: one vector of sampled blocks over the whole
       dimension range with the R = 1 kernels, seconds per vector interpolated
       in log(dim), times R per block.
@@ -192,9 +193,35 @@ def sweep_estimate(ftlm, R=128, m=80, pack_every=8, log=None):
         pv = time.perf_counter() - a
         plain_vec.append((d, pv))
         say(f"# sweep: plain vector {b[:4]} dim {d}: {pv * 1000:.2f} ms")
-    eng.close()
     solo_rate.sort()
     plain_vec.sort()
+    # check of the interpolation on solo blocks outside the sample
+    checks = []
+    if validate > 0:
+        import numpy as np
+
+        rng = np.random.default_rng(7)
+        others = [b for b in solo if b not in pick]
+        for j in rng.choice(len(others), size=min(validate, len(others)), replace=False):
+            b = others[int(j)]
+            eng.sector(*b[:2])
+            d = eng.block(b[2], b[3])
+            eng.prepare(R, ftlm.block_seed(12345, *b[:4]))
+            s = eng.run(R, m, ftlm.REDUCE_TREE)
+            a = time.perf_counter()
+            eng.finish(R, m)
+            secs = s["ms_total"] / 1000.0 + time.perf_counter() - a
+            pred = algo_bytes(b) / _interp_log([x[0] for x in solo_rate], [x[1] for x in solo_rate], d)
+            a = time.perf_counter()
+            eng.run(R, m, ftlm.REDUCE_TREE, per_vector_column=0)
+            pv = time.perf_counter() - a
+            pv_pred = _interp_log([x[0] for x in plain_vec], [x[1] for x in plain_vec], d)
+            checks.append({"block": list(b[:4]), "dim": d, "batched_s": round(secs, 3),
+                           "predicted_s": round(pred, 3), "plain_ms": round(pv * 1000, 2),
+                           "plain_predicted_ms": round(pv_pred * 1000, 2)})
+            say(f"# sweep check {b[:4]} dim {d}: batched {secs:.3f}s vs predicted {pred:.3f}s; "
+                f"plain {pv * 1000:.1f} ms vs predicted {pv_pred * 1000:.1f} ms")
+    eng.close()
     t_solo = sum(algo_bytes(b) / _interp_log([x[0] for x in solo_rate], [x[1] for x in solo_rate], b[4])
                  for b in solo)
     t_plain_ftlm = sum(R * _interp_log([x[0] for x in plain_vec], [x[1] for x in plain_vec], b[4])
@@ -210,7 +237,7 @@ def sweep_estimate(ftlm, R=128, m=80, pack_every=8, log=None):
                        "plain_sampled": len(plain_pick) + len(pick)},
             "solo_rate_TBps": [(d, round(r / 1e12, 3)) for d, r in solo_rate],
             "plain_ms_per_vector": [(d, round(t * 1000, 2)) for d, t in plain_vec],
-            "start_vectors_s_sampled": round(t_host_starts, 2)}
+            "start_vectors_s_sampled": round(t_host_starts, 2), "checks": checks}
 
 
 def read_traffic():
@@ -235,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the whole-sweep estimate (batched vs plain, from a stratified block sample)")
+    ap.add_argument("--sweep-validate", type=int, default=0, metavar="N",
+                    help="also time N random solo blocks outside the sample against the estimate's prediction")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
@@ -316,7 +345,8 @@ def main():
     sweep = None
     if not args.no_sweep:
         try:
-            sweep = sweep_estimate(ftlm, R, m, log=lambda t: print(t, file=sys.stderr, flush=True))
+            sweep = sweep_estimate(ftlm, R, m, validate=args.sweep_validate,
+                                   log=lambda t: print(t, file=sys.stderr, flush=True))
         except Exception as e:  # reported, not hidden: the per-block measurement above stands
             sweep = {"error": f"{type(e).__name__}: {e}"}
     cpu = None if args.no_cpu else cpu_baseline()
