@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -61,3 +62,28 @@ def test_create_fails_loudly_without_gpu():
     with pytest.raises(pkg.FtlmError) as e:
         pkg.Engine(lx=2, ly=2)
     assert "no HIP device" in str(e.value) and "-2" in str(e.value)
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="a GPU is visible here")
+def test_surrogate_fails_loudly_without_gpu():
+    from ftlm_extended_hubbard_development_amd import FtlmError, surrogate
+
+    with pytest.raises(FtlmError) as e:
+        surrogate.Surrogate(batch=64)
+    assert "no HIP device" in str(e.value) and "-2" in str(e.value)
+
+
+def test_surrogate_config_errors_before_device():
+    """Argument errors are reported before any device is touched."""
+    import ctypes as C
+
+    import ftlm_extended_hubbard_development_amd as pkg
+
+    lib = pkg.load_library()
+    h = C.c_void_p()
+    for batch, scale in [(100, 1.0), (0, 1.0), (64, 0.0)]:
+        cfg = pkg.FtlmMlpConfig(batch, 1e-3, 0.9, 0.999, 1e-8, (C.c_double * 2)(0, 0), (C.c_double * 2)(scale, 1))
+        assert lib.ftlm_mlp_create(C.byref(cfg), 0, C.byref(h)) == -1
+    x = np.zeros((3, 7))
+    f = np.zeros((3, 32), np.float32)
+    assert lib.ftlm_mlp_encode(3, x.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_float))) == 0
