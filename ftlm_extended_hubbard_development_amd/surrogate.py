@@ -20,6 +20,10 @@ Command line::
     python -m ftlm_extended_hubbard_development_amd.surrogate generate --lattice 3x3 --draws 48 --out d33.npz
     python -m ftlm_extended_hubbard_development_amd.surrogate train d42.npz d33.npz --out model.npz
     python -m ftlm_extended_hubbard_development_amd.surrogate scan model.npz --lattice 3x3 --grid 16 ...
+    python -m ftlm_extended_hubbard_development_amd.surrogate gain pretrained:4x2 --lattice 4x2 --u 8 --exact
+
+A model argument ``pretrained:LATTICE`` names the parameters shipped in
+``data/`` (``PRETRAINED``).
 """
 import argparse
 import ctypes as C
@@ -40,6 +44,17 @@ RANGES = {"u": (4.0, 10.0), "v": (0.0, 2.0), "tp": (-0.3, 0.3), "phix": (0.0, 1.
           "beta": (2.0, 20.0), "mu": (-5.0, 15.0)}
 INPUTS = ("u", "v", "tp", "phix", "phiy", "beta", "mu")
 TOL_N = 1e-3  # |dn| above which FTLM replaces the surrogate
+
+# trained parameters shipped with the package (the command that made each
+# is in its "meta" entry)
+PRETRAINED = {"4x2": os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_4x2.npz")}
+
+
+def pretrained(lattice, device=0, batch=4096):
+    """The shipped surrogate of a lattice ("4x2"), loaded on the GPU."""
+    if lattice not in PRETRAINED:
+        raise ValueError(f"no pretrained surrogate for {lattice!r}; have {sorted(PRETRAINED)}")
+    return Surrogate.load(PRETRAINED[lattice], device=device, batch=batch)
 
 
 def layout():
@@ -425,13 +440,21 @@ def evaluate(sur, x, y, tol=TOL_N):
 
 
 def gain(sur, lattice, u, v, tp, grid, betas, judge_stride=4, tol=TOL_N, samples=32, steps=80, finetune_steps=2000,
-         mu_min=-5.0, mu_max=15.0, mu_count=81, log=None):
+         mu_min=-5.0, mu_max=15.0, mu_count=81, save=None, exact=False, low_samples=(), log=None):
     """What the surrogate gains on a dense twist scan: the judged scan
     (surrogate + FTLM judges + FTLM fallback) against FTLM on every twist
     (the method without the surrogate). Returns times, the number of twists
     the surrogate served and the error of the served values against the
     all-FTLM scan (judge values are FTLM's own, so only surrogate-served rows
-    can differ)."""
+    can differ). interp_check_err is the error of the plain mean of FTLM at
+    each check twist's four cell corners, the part of the check error that
+    comes from the cell size rather than from the network. With `save`, the
+    scan's rows (all-FTLM, surrogate and served values) go to that .npz.
+    With `exact`, every twist is also computed with every block diagonalised
+    exactly, and both scans are measured against that: FTLM's own sampling
+    error against the error of the judged scan at each tolerance (the gain
+    at equal error), and, for each R in `low_samples`, FTLM on every twist
+    with R random vectors (the cheaper scan without the network)."""
     mus = dict(mu_min=mu_min, mu_max=mu_max, mu_count=mu_count)
     r = scan(sur, lattice, u, v, tp, grid, betas, judge_stride=judge_stride, tol=tol, samples=samples, steps=steps,
              finetune_steps=finetune_steps, log=log, **mus)
@@ -449,14 +472,54 @@ def gain(sur, lattice, u, v, tp, grid, betas, judge_stride=4, tol=TOL_N, samples
     dn_sur = np.abs(ypred[:, :, 0] - yf.reshape(len(tw), nb_nm, 2)[:, :, 0]).max(axis=1)
     judged = set(r["fit"]) | set(r["check"])
     s = judge_stride
+    yex, t_exact = None, 0.0
+    if exact:
+        # every block diagonalised exactly: the scan's exact answer
+        lx, ly = parse_lattice(lattice)
+        _, yex, t_exact = ftlm_twists(lattice, u, v, tp, tw, betas, samples=samples, steps=steps, log=log,
+                                      extra=("--exact-block-threshold", str(4 ** (lx * ly))), **mus)
+        yex = yex.reshape(len(tw), nb_nm, 2)[:, :, 0]
+    yfn = yf.reshape(len(tw), nb_nm, 2)[:, :, 0]
     curve = []
     for t in sorted({tol, 2e-3, 5e-3, 1e-2, 2e-2, 5e-2}):
         ok = {((i % grid) // s, (i // grid) // s) for i, e in zip(r["check"], r["check_err"]) if e <= t}
         served = [i for i in range(len(tw)) if i not in judged and ((i % grid) // s, (i // grid) // s) in ok]
         n_fall = len(tw) - len(judged) - len(served)
         t_est = r["t_surrogate"] + r["t_judge"] + n_fall * t_twist
-        curve.append({"tol": t, "surrogate_twists": len(served), "speedup": t_full / t_est,
-                      "dn_max_served": float(max((dn_sur[i] for i in served), default=0.0))})
+        point = {"tol": t, "surrogate_twists": len(served), "speedup": t_full / t_est,
+                 "dn_max_served": float(max((dn_sur[i] for i in served), default=0.0))}
+        if yex is not None:
+            yt = yfn.copy()
+            yt[served] = ypred[served, :, 0]
+            d = np.abs(yt - yex)
+            point.update(dn_exact_max=float(d.max()), dn_exact_p50=float(np.median(d)))
+        curve.append(point)
+    # what the cell size alone allows: FTLM at the check twist against the
+    # mean of FTLM at its four cell corners (the twist is periodic in 1)
+    yg = yf.reshape(len(tw), nb_nm, 2)
+    interp_err = []
+    for i in r["check"]:
+        x0, y0 = (i % grid) - s // 2, (i // grid) - s // 2
+        corners = [((y0 + dy) % grid) * grid + (x0 + dx) % grid for dy in (0, s) for dx in (0, s)]
+        interp_err.append(float(np.abs(yg[corners, :, 0].mean(axis=0) - yg[i, :, 0]).max()))
+    if save:
+        extra = {} if yex is None else {"n_exact": yex.ravel()}
+        np.savez_compressed(save, x=r["x"], y_ftlm=yf, y_surrogate=r["y_surrogate"], y_served=r["y"],
+                            twist_source=r["twist_source"], check=np.asarray(r["check"]),
+                            check_err=r["check_err"], grid=grid, judge_stride=s, **extra)
+    exact_out = {}
+    if yex is not None:
+        d = np.abs(yfn - yex)
+        exact_out = {"t_exact": t_exact, "ftlm_dn_exact_max": float(d.max()),
+                     "ftlm_dn_exact_p50": float(np.median(d))}
+        # the method without the network made cheaper the usual way: fewer
+        # random vectors on every twist
+        for rl in low_samples:
+            _, yl, tl = ftlm_twists(lattice, u, v, tp, tw, betas, samples=rl, steps=steps, log=log, **mus)
+            d = np.abs(yl.reshape(len(tw), nb_nm, 2)[:, :, 0] - yex)
+            exact_out.setdefault("ftlm_fewer_samples", []).append(
+                {"samples": rl, "t": tl, "speedup": t_full / tl, "dn_exact_max": float(d.max()),
+                 "dn_exact_p50": float(np.median(d))})
     return {"twists": len(tw), "judges": len(judged), "fallback": len(r["fallback"]),
             "surrogate_twists": int((r["twist_source"] == 0).sum()),
             "t_ftlm_all": t_full, "t_scan": t_scan, "t_surrogate": r["t_surrogate"], "t_judge": r["t_judge"],
@@ -464,10 +527,17 @@ def gain(sur, lattice, u, v, tp, grid, betas, judge_stride=4, tol=TOL_N, samples
             "dn_max_served": float(dn.max()),
             "dn_max_surrogate_rows": float(dn[sur_rows].max()) if sur_rows.any() else 0.0,
             "dn_p50_surrogate_rows": float(np.quantile(dn[sur_rows], 0.5)) if sur_rows.any() else 0.0,
-            "check_err": [float(e) for e in r["check_err"]], "tolerance_curve": curve}
+            "check_err": [float(e) for e in r["check_err"]], "interp_check_err": interp_err,
+            "tolerance_curve": curve, **exact_out}
 
 
 # ---------------------------------------------------------------------- CLI
+def _load_model(name):
+    if name.startswith("pretrained:"):
+        return pretrained(name.split(":", 1)[1])
+    return Surrogate.load(name)
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(prog="surrogate", description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -506,6 +576,10 @@ def _main(argv=None):
         s.add_argument("--finetune-steps", type=int, default=2000)
         if name == "scan":
             s.add_argument("--out", required=True)
+        else:
+            s.add_argument("--save", default=None)
+            s.add_argument("--exact", action="store_true")
+            s.add_argument("--low-samples", default="", help="with --exact: FTLM scans at these R, e.g. 4,8")
     a = ap.parse_args(argv)
 
     def log(m):
@@ -528,17 +602,18 @@ def _main(argv=None):
         sur.save(a.out, meta={"data": a.data, "steps": a.steps, "train_s": time.perf_counter() - t0})
         log(f"wrote {a.out}: final loss {float(hist[-50:].mean()):.3e}")
     elif a.cmd == "evaluate":
-        sur = Surrogate.load(a.model)
+        sur = _load_model(a.model)
         for p in a.data:
             d = np.load(p, allow_pickle=False)
             log(json.dumps({"data": p, **evaluate(sur, d["x"], d["y"])}))
     else:
-        sur = Surrogate.load(a.model)
+        sur = _load_model(a.model)
         kw = dict(judge_stride=a.judge_stride, tol=a.tol, samples=a.samples, steps=a.steps,
                   finetune_steps=a.finetune_steps, log=log)
         betas = [float(b) for b in a.betas.split(",")]
         if a.cmd == "gain":
-            log(json.dumps(gain(sur, a.lattice, a.u, a.v, a.tp, a.grid, betas, **kw)))
+            log(json.dumps(gain(sur, a.lattice, a.u, a.v, a.tp, a.grid, betas, save=a.save, exact=a.exact,
+                                  low_samples=[int(r) for r in a.low_samples.split(",") if r], **kw)))
         else:
             r = scan(sur, a.lattice, a.u, a.v, a.tp, a.grid, betas, **kw)
             np.savez(a.out, **{k: np.asarray(v) for k, v in r.items()})

@@ -11,6 +11,7 @@ Pins of the oracle:
     random coordinates of every tensor (independent of torch);
   * Adam == torch.optim.Adam over several steps.
 """
+import json
 import os
 import subprocess
 import sys
@@ -191,3 +192,17 @@ def test_draws_are_seeded_and_in_range():
     assert a == surrogate.draw_models(20, 3) and a != surrogate.draw_models(20, 4)
     for u, v, tp in a:
         assert 4 <= u <= 10 and 0 <= v <= 2 and -0.3 <= tp <= 0.3
+
+
+def test_pretrained_parameters_ship_with_the_package():
+    """The shipped 4x2 network loads without unpickling and fits the engine's
+    layout; its normalisation is finite and its provenance recorded."""
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    d = np.load(surrogate.PRETRAINED["4x2"], allow_pickle=False)
+    assert d["params"].dtype == np.float32 and d["params"].size == surrogate.param_count()
+    assert np.isfinite(d["params"]).all() and (d["y_scale"] > 0).all()
+    meta = json.loads(str(d["meta"]))
+    assert meta["lattice"] == "4x2" and "generate" in meta["data"]
+    with pytest.raises(ValueError, match="no pretrained surrogate"):
+        surrogate.pretrained("5x5")
