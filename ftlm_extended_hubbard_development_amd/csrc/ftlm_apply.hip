@@ -676,22 +676,40 @@ __global__ void __launch_bounds__(256) k_apply_hl(int64_t dim, int R, int W, int
     }
     const int64_t e0 = row_ptr[row], e1 = row_ptr[row + 1];
     int64_t e = e0;
-    for (; e + 4 <= e1; e += 4) {
-      int32_t cl[4];
-      cd cf[4];
-      double2 xv[4][VPL];
+    // the next group's hop entries load while this group's rows are in
+    // flight (summation order unchanged); a partial last group goes hop by
+    // hop (masking it costs more in branches than it saves here)
+    int32_t ncl[4];
+    uint32_t ncode[4];
+    if (e + 4 <= e1) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        cl[q] = hcol[e + q];
-        cf[q] = hop_coef(hcode[e + q], ptab, spec);
+        ncl[q] = hcol[e + q];
+        ncode[q] = hcode[e + q];
       }
+    }
+    for (; e + 4 <= e1; e += 4) {
+      double2 xv[4][VPL];
+      uint32_t code[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q) {
+        code[q] = ncode[q];
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
           const int cc = lane + 64 * v;
-          xv[q][v] = cc < wj ? x[(int64_t)cl[q] * R + cb + cc] : make_double2(0.0, 0.0);
+          xv[q][v] = cc < wj ? x[(int64_t)ncl[q] * R + cb + cc] : make_double2(0.0, 0.0);
         }
+      }
+      if (e + 8 <= e1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ncl[q] = hcol[e + 4 + q];
+          ncode[q] = hcode[e + 4 + q];
+        }
+      }
+      cd cf[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cf[q] = hop_coef(code[q], ptab, spec);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll

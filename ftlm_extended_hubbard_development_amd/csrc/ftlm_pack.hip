@@ -19,6 +19,7 @@ namespace ftlm {
 namespace {
 
 constexpr int kPackThreads = 512;
+constexpr int kG = 4;  // hops per gather group in the apply loop
 constexpr int kPackWaves = kPackThreads / 64;
 constexpr uint32_t kNegP = 1u << 31, kSpecialP = 1u << 30;
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ double col_norm2(const double2* __restrict__ x, int64
 // per thread (thread p of a column owns rows p, p + NP, ...), and the NP
 // partials of a column are added in index order: two vector passes fewer
 // per step and no serial phase; a fixed order, so runs repeat exactly.
-template <int W>
+template <int W, bool TREE>
 __global__ void __launch_bounds__(kPackThreads, 4) k_pack_lanczos(PackArgs a) {
   constexpr int RPW = 64 / W;             // rows per wave pass
   constexpr int NP = kPackWaves * RPW;    // threads per column
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(kPackThreads, 4) k_pack_lanczos(PackArgs a) {
   const int lane = tid & 63, wave = tid >> 6;
   const int c = lane & (W - 1);
   const int rs = lane / W;
-  const bool tree = a.tree != 0;
+  constexpr bool tree = TREE;  // a.tree, fixed per instantiation
   double2* x0 = a.slots + (int64_t)blockIdx.x * 3 * a.slot_stride;
   double2* x1 = x0 + a.slot_stride;
   double2* y = x1 + a.slot_stride;
@@ -142,23 +143,37 @@ __global__ void __launch_bounds__(kPackThreads, 4) k_pack_lanczos(PackArgs a) {
         cd acc = {0.0 + dg * xs.x, 0.0 + dg * xs.y};
         const int64_t e0 = e.row_ptr[i], e1 = e.row_ptr[i + 1];
         int64_t h = e0;
-        for (; h + 4 <= e1; h += 4) {
-          int32_t cl[4];
-          cd cf[4];
-          double2 xv[4];
+        // hops go in groups of kG (the last one partial): the next group's
+        // hop entries load while this group's vector rows are in flight; the
+        // sums keep the hop order
+        int32_t ncl[kG];
+        uint32_t ncode[kG];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            cl[q] = e.col[h + q];
-            cf[q] = pack_coef(e.code[h + q], e.ptab, e.spec);
+        for (int q = 0; q < kG; ++q) {
+          ncl[q] = h + q < e1 ? e.col[h + q] : 0;
+          ncode[q] = h + q < e1 ? e.code[h + q] : 0u;
+        }
+        for (; h < e1; h += kG) {
+          const int n = e1 - h < kG ? (int)(e1 - h) : kG;
+          double2 xv[kG];
+          uint32_t code[kG];
+#pragma unroll
+          for (int q = 0; q < kG; ++q) {
+            xv[q] = q < n ? x0[(int64_t)ncl[q] * W + c] : make_double2(0.0, 0.0);
+            code[q] = ncode[q];
           }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) xv[q] = x0[(int64_t)cl[q] * W + c];
+          for (int q = 0; q < kG; ++q) {
+            const int64_t hn = h + kG + q;
+            ncl[q] = hn < e1 ? e.col[hn] : 0;
+            ncode[q] = hn < e1 ? e.code[hn] : 0u;
+          }
+          cd cf[kG];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc = cadd(acc, cmul(cf[q], {xv[q].x, xv[q].y}));
-        }
-        for (; h < e1; ++h) {
-          const double2 t = x0[(int64_t)e.col[h] * W + c];
-          acc = cadd(acc, cmul(pack_coef(e.code[h], e.ptab, e.spec), {t.x, t.y}));
+          for (int q = 0; q < kG; ++q) cf[q] = q < n ? pack_coef(code[q], e.ptab, e.spec) : cd{0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < kG; ++q)
+            if (q < n) acc = cadd(acc, cmul(cf[q], {xv[q].x, xv[q].y}));
         }
         y[i * W + c] = make_double2(acc.re, acc.im);
         if (tree) dpart = dpart + (xs.x * acc.re + xs.y * acc.im);
@@ -245,13 +260,18 @@ void launch_pack_reset(const PackArgs& a, int rcap, hipStream_t s) {
   k_pack_reset<<<(unsigned)std::max<int64_t>(1, std::min<int64_t>((rcap + 255) / 256, 1024)), 256, 0, s>>>(a, rcap);
 }
 
+template <int W, bool TREE>
+void launch_pack_w(const PackArgs& a, int grid, hipStream_t s) {
+  k_pack_lanczos<W, TREE><<<grid, kPackThreads, 0, s>>>(a);
+}
+
 void launch_pack_lanczos(const PackArgs& a, int W, int grid, hipStream_t s) {
   switch (W) {
-    case 4: k_pack_lanczos<4><<<grid, kPackThreads, 0, s>>>(a); break;
-    case 8: k_pack_lanczos<8><<<grid, kPackThreads, 0, s>>>(a); break;
-    case 16: k_pack_lanczos<16><<<grid, kPackThreads, 0, s>>>(a); break;
-    case 32: k_pack_lanczos<32><<<grid, kPackThreads, 0, s>>>(a); break;
-    default: k_pack_lanczos<64><<<grid, kPackThreads, 0, s>>>(a); break;
+    case 4: a.tree ? launch_pack_w<4, true>(a, grid, s) : launch_pack_w<4, false>(a, grid, s); break;
+    case 8: a.tree ? launch_pack_w<8, true>(a, grid, s) : launch_pack_w<8, false>(a, grid, s); break;
+    case 16: a.tree ? launch_pack_w<16, true>(a, grid, s) : launch_pack_w<16, false>(a, grid, s); break;
+    case 32: a.tree ? launch_pack_w<32, true>(a, grid, s) : launch_pack_w<32, false>(a, grid, s); break;
+    default: a.tree ? launch_pack_w<64, true>(a, grid, s) : launch_pack_w<64, false>(a, grid, s); break;
   }
 }
 
