@@ -600,12 +600,18 @@ __device__ __forceinline__ cd hop_coef(uint32_t code, const double2* __restrict_
   return (code & kNeg) ? cd{-v.x, -v.y} : cd{v.x, v.y};
 }
 
+// a wave-uniform 64-bit value, kept in scalar registers
+__device__ __forceinline__ int64_t uniform64(int64_t v) {
+  return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v)) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32)) << 32));
+}
+
 // Batched gather over a hop list: one wavefront per row, lane l owns
 // columns cb + l + 64 v (v < VPL); the row's hop entries are wave-uniform
 // and read with scalar loads. With W < R the columns are tiled as in
 // k_apply_tiled (items = (group, column tile), XCD-contiguous).
 template <int VPL>
-__global__ void __launch_bounds__(256) k_apply_hl(int64_t dim, int R, int W, int J,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 2 ? 8 : 4))) k_apply_hl(int64_t dim, int R, int W, int J,
                                                    const int64_t* __restrict__ offs, int n_canon,
                                                    const int64_t* __restrict__ row_ptr,
                                                    const int32_t* __restrict__ hcol,
@@ -635,6 +641,14 @@ __global__ void __launch_bounds__(256) k_apply_hl(int64_t dim, int R, int W, int
     u1 = u0 + chunk < nunits ? u0 + chunk : nunits;
     ufirst = u0 + (int64_t)l * 4 + wave;
     ustep = (int64_t)nwg * 4;
+  }
+  // untiled: the row_ptr pair of the wave's next row loads one row ahead
+  // (the unit sequence is wave-uniform)
+  ufirst = uniform64(ufirst);
+  int64_t nxe0 = 0, nxe1 = 0;
+  if (J == 1 && ufirst < u1) {
+    nxe0 = row_ptr[ufirst];
+    nxe1 = row_ptr[ufirst + 1];
   }
   for (int64_t u = ufirst; u < u1; u += ustep) {
     int64_t row;
@@ -674,7 +688,19 @@ __global__ void __launch_bounds__(256) k_apply_hl(int64_t dim, int R, int W, int
       }
       acc[v] = {0.0 + dg * xs[v].re, 0.0 + dg * xs[v].im};
     }
-    const int64_t e0 = row_ptr[row], e1 = row_ptr[row + 1];
+    int64_t e0, e1;
+    if (J == 1) {
+      e0 = nxe0;
+      e1 = nxe1;
+      const int64_t un = u + ustep;
+      if (un < u1) {
+        nxe0 = row_ptr[un];
+        nxe1 = row_ptr[un + 1];
+      }
+    } else {
+      e0 = row_ptr[row];
+      e1 = row_ptr[row + 1];
+    }
     int64_t e = e0;
     // the next group's hop entries load while this group's rows are in
     // flight (summation order unchanged); a partial last group goes hop by
