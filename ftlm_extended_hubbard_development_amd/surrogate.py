@@ -455,6 +455,8 @@ def gain(sur, lattice, u, v, tp, grid, betas, judge_stride=4, tol=TOL_N, samples
     error against the error of the judged scan at each tolerance (the gain
     at equal error), and, for each R in `low_samples`, FTLM on every twist
     with R random vectors (the cheaper scan without the network)."""
+    if low_samples and not exact:
+        raise ValueError("low_samples scans are measured against the exact answer: pass exact=True")
     mus = dict(mu_min=mu_min, mu_max=mu_max, mu_count=mu_count)
     r = scan(sur, lattice, u, v, tp, grid, betas, judge_stride=judge_stride, tol=tol, samples=samples, steps=steps,
              finetune_steps=finetune_steps, log=log, **mus)

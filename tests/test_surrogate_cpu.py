@@ -206,3 +206,12 @@ def test_pretrained_parameters_ship_with_the_package():
     assert meta["lattice"] == "4x2" and "generate" in meta["data"]
     with pytest.raises(ValueError, match="no pretrained surrogate"):
         surrogate.pretrained("5x5")
+
+
+def test_gain_low_samples_needs_exact():
+    """The fewer-sample FTLM scans are only reported against the exact answer;
+    asking for them without it is an argument error raised before any work."""
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    with pytest.raises(ValueError, match="exact=True"):
+        surrogate.gain(None, "4x2", 8.0, 0.5, 0.1, 8, [2.0], low_samples=[16])
