@@ -40,17 +40,25 @@ __device__ double hypot_safe(double a, double b) {
 
 // Implicit QL with Wilkinson shifts on a symmetric tridiagonal matrix
 // (d: diagonal, e[i]: coupling of i and i+1). Returns false if an eigenvalue
-// needed more than 60 sweeps.
+// needed more than 60 sweeps. A coupling splits the matrix when it is below
+// eps times its two diagonal neighbours or below eps times the matrix norm:
+// the second test (a perturbation of eps ||T||, the size of the reduction's
+// own rounding) is what lets clusters of eigenvalues near zero converge,
+// where the relative test alone waits for couplings below rounding noise.
 __device__ bool tridiag_ql(double* d, double* e, int n) {
   if (n <= 1) return true;
   e[n - 1] = 0.0;
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i)
+    tnorm = fmax(tnorm, fabs(d[i]) + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+  const double floor_e = 2.220446049250313e-16 * tnorm;
   for (int l = 0; l < n; ++l) {
     int iter = 0;
     int m;
     do {
       for (m = l; m < n - 1; ++m) {
         const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+        if (fabs(e[m]) <= 2.220446049250313e-16 * dd || fabs(e[m]) <= floor_e) break;
       }
       if (m != l) {
         if (++iter > 60) return false;

@@ -584,6 +584,11 @@ __global__ void k_ritz_ql(int R, int steps, const double* __restrict__ alpha, co
     e[i] = i + 1 < n ? beta[(int64_t)i * R + r] : 0.0;
     z[i] = i == 0 ? 1.0 : 0.0;
   }
+  // splitting: relative to the neighbouring diagonals, or below eps ||T||
+  // (lets clusters of Ritz values near zero converge)
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(d[i]) + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+  const double floor_e = 2.220446049250313e-16 * tnorm;
   bool ok = true;
   for (int l = 0; l < n && ok; ++l) {
     int iter = 0;
@@ -591,7 +596,7 @@ __global__ void k_ritz_ql(int R, int steps, const double* __restrict__ alpha, co
     do {
       for (m = l; m < n - 1; ++m) {
         const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+        if (fabs(e[m]) <= 2.220446049250313e-16 * dd || fabs(e[m]) <= floor_e) break;
       }
       if (m != l) {
         if (++iter > 60) {

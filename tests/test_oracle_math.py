@@ -293,3 +293,20 @@ def test_thermo_closed_form(orc):
         assert abs(got_logz - math.log(Z)) < 1e-12
         assert abs(st["n1"][k] / st["z"][k] - N1 / Z) < 1e-12
         assert abs(st["n2"][k] / st["z"][k] - N2 / Z) < 1e-11
+
+
+@pytest.mark.parametrize("nup,ndn", [(3, 5), (4, 4)])
+def test_hermitian_eigensolver_zero_clusters(orc, nup, ndn):
+    """Free fermions (U = V = 0) on the twisted 4x2 lattice: the k = (0, 0)
+    blocks of these sectors (dims 392 and 628) hold clusters of eigenvalues
+    at zero, where a QL split test relative to the neighbouring diagonals
+    alone never fires (the couplings stall at rounding noise). The solver
+    must converge and agree with numpy."""
+    m = oracle_lib.model(lx=4, ly=2, tp=0.13, phix=0.3, phiy=0.2, u=0.0, v=0.0)
+    blk = orc.block(m, nup, ndn, 0, 0)
+    A = blk.dense()
+    A = 0.5 * (A + A.conj().T)
+    ref = np.linalg.eigvalsh(A)
+    assert (np.abs(ref) < 1e-10).sum() >= 8
+    w = orc.herm_eigvals(A)
+    assert np.max(np.abs(w - ref)) < 1e-12 * max(1.0, np.abs(ref).max())
