@@ -608,20 +608,17 @@ __device__ __forceinline__ int64_t uniform64(int64_t v) {
 
 // Batched gather over a hop list: one wavefront per row, lane l owns
 // columns cb + l + 64 v (v < VPL); the row's hop entries are wave-uniform
-// and read with scalar loads. With W < R the columns are tiled as in
-// k_apply_tiled (items = (group, column tile), XCD-contiguous).
+// and read with scalar loads, the next group of four and the next row's
+// row_ptr pair one step ahead. With W < R the columns are tiled as in
+// k_apply_tiled (items = (group, column tile), XCD-contiguous). Up to 128
+// columns (VPL <= 2) it is held to 8 waves per SIMD.
 template <int VPL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 2 ? 8 : 4))) k_apply_hl(int64_t dim, int R, int W, int J,
-                                                   const int64_t* __restrict__ offs, int n_canon,
-                                                   const int64_t* __restrict__ row_ptr,
-                                                   const int32_t* __restrict__ hcol,
-                                                   const uint32_t* __restrict__ hcode,
-                                                   const double2* __restrict__ ptab,
-                                                   const double2* __restrict__ spec,
-                                                   const double* __restrict__ diag,
-                                                   const double2* __restrict__ x, double2* __restrict__ y,
-                                                   double* __restrict__ partials, unsigned* counter,
-                                                   double* __restrict__ dot_out) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 2 ? 8 : 4)))
+k_apply_hl(int64_t dim, int R, int W, int J, const int64_t* __restrict__ offs, int n_canon,
+           const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ hcol, const uint32_t* __restrict__ hcode,
+           const double2* __restrict__ ptab, const double2* __restrict__ spec, const double* __restrict__ diag,
+           const double2* __restrict__ x, double2* __restrict__ y, double* __restrict__ partials, unsigned* counter,
+           double* __restrict__ dot_out) {
   __shared__ double wpart[4][64 * VPL];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -672,8 +669,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <=
       row = offs[lo] + (k - (int64_t)j * gsz);
     }
     // the row is wave-uniform: keep it (and so the hop metadata loads) scalar
-    row = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)row)) |
-                    ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(row >> 32)) << 32));
+    row = uniform64(row);
     const int cb = j * W;
     const int wj = cb + W <= R ? W : R - cb;
     const double dg = diag[row];
