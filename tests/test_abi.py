@@ -87,3 +87,16 @@ def test_surrogate_config_errors_before_device():
     x = np.zeros((3, 7))
     f = np.zeros((3, 32), np.float32)
     assert lib.ftlm_mlp_encode(3, x.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_float))) == 0
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU; asking for more exits with a message (and
+    touches no device)."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
