@@ -2,11 +2,12 @@
 
 Workload: the largest block of the 4x4 sweep (lx=ly=4, t=1, t'=0, U=7, V=0):
 (N_up, N_down, k) = (8, 8, (0, 0)), dim = 10,353,252, with all R = 128
-seeded random vectors advancing together for 80 Lanczos steps (tree
-reductions). One bench step is one FTLM pass over that block: 80 batched
-H applies + recurrences; the start vectors are resident in HBM before the
-timed region (the host generates them once, bit-identical to the reference's
-seeded stream).
+seeded random vectors advancing together for 80 Lanczos steps. One bench
+step is what the drop-in executable runs for that block (ftlm_ftlm_block on
+its default path): the seeded start vectors drawn on the GPU (the reference's
+mt19937_64 stream), 80 batched H applies + tree-order recurrences, and the
+QL Ritz values and weights copied back. The block's tables and hop list are
+resident in HBM before the timed region.
 
 value = algorithmic H-apply bandwidth: sum over applies of
         dim * R * 16 B * (hbar + 2)  /  summed H-apply kernel time
@@ -50,6 +51,15 @@ def cpu_baseline(budget_s=25.0):
         lib = oracle_lib.oracle()
         kind = "port"
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     model = oracle_lib.model(lx=4, ly=4, u=7.0)
     # a mid-size block of the same sweep at the workload's R and steps
     # (~20 s on 16 cores; the reference's serial basis build is not timed)
@@ -63,7 +73,7 @@ def cpu_baseline(budget_s=25.0):
     dt = time.perf_counter() - t0
     m = min(steps, blk.dim)
     byts = m * blk.dim * R * 16.0 * (hbar(16, nup, ndn, 4) + 2.0)
-    return {"value": round(byts / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+    return {"value": round(byts / dt / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind, "cpu": cpu_model,
             "sample": (f"4x4 U=7 block (Nup={nup},Ndown={ndn},k=({mx},{my})) dim={blk.dim}, R={R}, {steps} steps, "
                        f"run_ftlm_block incl. Jacobi, {dt:.1f} s on {threads} threads")}
 
@@ -300,9 +310,11 @@ def main():
     print(f"# block ({nup},{ndn},{mx},{my}) dim={dim} basis {t_basis:.2f}s", file=sys.stderr, flush=True)
     R, m = args.R, args.lanczos_steps
     bseed = ftlm.block_seed(12345, nup, ndn, mx, my)
+    # the executable's default: start vectors drawn on the GPU (tree mode)
+    eng.set_start_source(ftlm.START_DEVICE)
     t0 = time.perf_counter()
     eng.prepare(R, bseed)
-    print(f"# start vectors resident ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    print(f"# start vectors for tuning ({time.perf_counter() - t0:.2f}s)", file=sys.stderr, flush=True)
     if args.tile == "auto":
         tile, tile_ms, tune = eng.autotune(R)
         print(f"# autotune: {tune} -> {tile}", file=sys.stderr, flush=True)
@@ -312,26 +324,29 @@ def main():
         tile, tune = eng.apply_config(R), None
 
     for w in range(args.warmup):
-        eng.run(R, m, ftlm.REDUCE_TREE)
+        eng.ftlm_block(R, m, bseed, reduce=ftlm.REDUCE_TREE)
         print(f"# warmup {w} done", file=sys.stderr, flush=True)
     ms_apply = 0.0
     bytes_apply = 0.0
     n_apply = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
-        st = eng.run(R, m, ftlm.REDUCE_TREE)
+        t1 = time.perf_counter()
+        eig, wts = eng.ftlm_block(R, m, bseed, reduce=ftlm.REDUCE_TREE)
+        dt = time.perf_counter() - t1
+        st = eng.last_stats()
         ms_apply += st["ms_apply"]
         bytes_apply += st["bytes_apply"]
         n_apply += st["n_apply"]
-        print(f"# step {k}: apply {st['ms_apply'] / st['n_apply']:.2f} ms/launch, run {st['ms_total']:.0f} ms",
-              file=sys.stderr, flush=True)
+        assert abs(wts.sum() - 1.0) < 1e-10, wts.sum()
+        print(f"# step {k}: apply {st['ms_apply'] / st['n_apply']:.2f} ms/launch, Lanczos {st['ms_total']:.0f} ms, "
+              f"block {dt * 1000:.0f} ms", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
     achieved = bytes_apply / (ms_apply / 1000.0) / 1e9
-    eig, wts = eng.finish(R, m)
-    assert abs(wts.sum() - 1.0) < 1e-10, wts.sum()
 
     # plain per-vector GPU FTLM on the same block (R = 1 kernels)
+    eng.prepare(R, bseed)
     plain_ms = []
     for c in range(args.plain_vectors):
         t1 = time.perf_counter()
@@ -366,7 +381,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic: seeded unit-norm complex Gaussian start vectors (reference stream, seed 12345)",
         "config": {"workload": f"4x4 U=7 FTLM block (Nup={nup},Ndown={ndn},k=({mx},{my})) dim={dim}, "
-                               f"R={R} batched vectors, {m} Lanczos steps, tree reductions",
+                               f"R={R} batched vectors, {m} Lanczos steps, the executable's default path "
+                               f"(GPU start vectors, tree reductions, QL Ritz)",
                    "lattice": "4x4", "u": 7.0, "R": R, "lanczos_steps": m, "block_dim": dim,
                    "apply_config": {"path": ["auto", "hoplist", "onthefly"][tile[0]], "tile": tile[1]},
                    "autotune_ms_per_apply": tune},

@@ -247,6 +247,14 @@ class Engine:
                                              _p(w, _dp), C.byref(cnt)))
         return e[: cnt.value], w[: cnt.value]
 
+    def last_stats(self):
+        """Device-event timings of the last Lanczos run (ftlm_run or the
+        batched runs of ftlm_block)."""
+        s = FtlmStats()
+        self._check(self.lib.ftlm_last_stats(self.h, C.byref(s)))
+        return {"ms_apply": s.ms_apply, "ms_total": s.ms_total, "n_apply": s.n_apply,
+                "bytes_apply": s.bytes_apply}
+
     def prepare(self, R, bseed):
         self._check(self.lib.ftlm_prepare(self.h, R, bseed))
 

@@ -629,13 +629,13 @@ void ensure_vectors(ftlm_ctx* c, int R) {
   for (int i = 0; i < 3; ++i) c->vec[i].get(bytes);
 }
 
-// upload seeded start vectors of samples 0..R-1 into layout 0 ([dim][R] in
-// vec[0]) or layout 1 (sample-major [R][dim] in vec[0])
-void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout) {
+// upload seeded start vectors of samples sample0..sample0+R-1 into layout 0
+// ([dim][R] in vec[0]) or layout 1 (sample-major [R][dim] in vec[0])
+void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout, int sample0 = 0) {
   const int64_t dim = c->cur.dim;
   ensure_vectors(c, R);
   if (c->start_source == FTLM_START_DEVICE) {
-    launch_start_vectors(bseed, 0, 0, R, R, dim, layout, c->vec[0].ptr<double>(), c->start_norms.as<double>(R),
+    launch_start_vectors(bseed, sample0, 0, R, R, dim, layout, c->vec[0].ptr<double>(), c->start_norms.as<double>(R),
                          c->stream);
     HIPCHECK(hipGetLastError());
     sync(c);
@@ -653,7 +653,7 @@ void prepare_starts(ftlm_ctx* c, int R, uint64_t bseed, int layout) {
   double* stage = c->vec[2].ptr<double>();
   for (int r0 = 0; r0 < R; r0 += chunk) {
     const int cnt = std::min(chunk, R - r0);
-    gen_starts(bseed, r0, cnt, dim, host);
+    gen_starts(bseed, sample0 + r0, cnt, dim, host);
     if (layout == 1) {
       HIPCHECK(hipMemcpyAsync(x + 2 * (int64_t)r0 * dim, host, (size_t)cnt * dim * 16, hipMemcpyHostToDevice,
                               c->stream));
@@ -957,6 +957,22 @@ int64_t gather_ritz(ftlm_ctx* c, int R, int steps, double* eig, double* w) {
 void check_R(int R) {
   if (R < 1 || R > 1024) throw Error(FTLM_ERR_LIMIT, "R must be in [1, 1024]");
 }
+
+// Columns one batched launch covers: the apply kernels hold at most 4 x 64
+// columns per wavefront (hop list and on the fly) and their fused dot stages
+// 256 per workgroup. ftlm_ftlm_block runs larger R in batches of this many
+// samples (the samples are independent, main.cpp:1011-1014).
+constexpr int kMaxBatchR = 256;
+
+void check_batch_R(int R) {
+  if (R < 1 || R > kMaxBatchR)
+    throw Error(FTLM_ERR_LIMIT, "R must be in [1, 256] for one batched call (ftlm_ftlm_block splits larger R)");
+}
+
+// widest column tile each apply path handles (hop list: 4 x 64 columns per
+// wavefront; on-the-fly tiles: 2 x 64)
+constexpr int kMaxTileHopList = 256;
+constexpr int kMaxTileOnTheFly = 128;
 
 // ---- packed blocks ---------------------------------------------------------
 constexpr uint64_t kPackDefaultBudget = (uint64_t)64 << 30;
@@ -1383,7 +1399,7 @@ int ftlm_apply(ftlm_ctx* ctx, int R, int kernel, const double* x, double* y) {
   return guarded([&] {
     if (!ctx || !x || !y) throw Error(FTLM_ERR_ARG, "null argument");
     need_block(ctx);
-    check_R(R);
+    check_batch_R(R);
     if (kernel == 1 && R != 1) throw Error(FTLM_ERR_ARG, "kernel 1 handles R = 1 only");
     if (kernel != 0 && kernel != 1) throw Error(FTLM_ERR_ARG, "kernel must be 0 (configured) or 1 (one row per thread)");
     const int64_t n = ctx->cur.dim;
@@ -1440,7 +1456,8 @@ int ftlm_lanczos(ftlm_ctx* ctx, int R, int steps, const double* start, uint64_t 
   return guarded([&] {
     if (!ctx || !alpha || !beta || !m) throw Error(FTLM_ERR_ARG, "null argument");
     need_block(ctx);
-    check_R(R);
+    if (per_vector) check_R(R);
+    else check_batch_R(R);
     if (steps < 1) throw Error(FTLM_ERR_ARG, "--samples and --lanczos-steps must be positive.");
     const int64_t n = ctx->cur.dim;
     if (n == 0) throw Error(FTLM_ERR_ARG, "empty block");
@@ -1505,14 +1522,23 @@ int ftlm_ftlm_block(ftlm_ctx* ctx, int R, int steps, uint64_t block_seed, int re
       return;
     }
     HIPCHECK(hipSetDevice(ctx->device));
-    prepare_starts(ctx, R, block_seed, per_vector ? 1 : 0);
-    run_all(ctx, R, steps, reduce, per_vector != 0, false);
     const int kdim = (int)std::min<int64_t>(steps, ctx->cur.dim);
-    // serial runs keep the reference's Jacobi (bit-identical Ritz pairs);
-    // tree runs use QL. The state arrays are [steps][R]; rows >= kdim unused.
-    ritz_device(ctx, R, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(), ctx->st_mcount.ptr<int32_t>(),
-                R, reduce == FTLM_REDUCE_SERIAL ? 0 : 1);
-    const int64_t n = gather_ritz(ctx, R, kdim, eig, w);
+    // samples r0..r0+cnt-1 per batch; weights are divided by the full R
+    // (main.cpp:1090-1094) and concatenated in sample order. Batched runs
+    // record their H-apply device time (ftlm_last_stats).
+    ctx->stats = ftlm_stats{};
+    int64_t n = 0;
+    for (int r0 = 0; r0 < R; r0 += kMaxBatchR) {
+      const int cnt = std::min(kMaxBatchR, R - r0);
+      prepare_starts(ctx, cnt, block_seed, per_vector ? 1 : 0, r0);
+      run_all(ctx, cnt, steps, reduce, per_vector != 0, per_vector == 0);
+      // serial runs keep the reference's Jacobi (bit-identical Ritz pairs);
+      // tree runs use QL. The state arrays are [steps][cnt]; rows >= kdim
+      // unused.
+      ritz_device(ctx, cnt, kdim, ctx->st_alpha.ptr<double>(), ctx->st_beta.ptr<double>(),
+                  ctx->st_mcount.ptr<int32_t>(), R, reduce == FTLM_REDUCE_SERIAL ? 0 : 1);
+      n += gather_ritz(ctx, cnt, kdim, eig ? eig + n : nullptr, w ? w + n : nullptr);
+    }
     if (count) *count = n;
     ctx->prepared_R = 0;
   });
@@ -1522,7 +1548,7 @@ int ftlm_prepare(ftlm_ctx* ctx, int R, uint64_t block_seed) {
   return guarded([&] {
     if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
     need_block(ctx);
-    check_R(R);
+    check_batch_R(R);
     HIPCHECK(hipSetDevice(ctx->device));
     prepare_starts(ctx, R, block_seed, 0);
     // keep the start vectors resident so that every timed run starts from
@@ -1919,6 +1945,9 @@ int ftlm_set_apply_config(ftlm_ctx* ctx, int path, int W) {
     if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
     if (path < 0 || path > 2) throw Error(FTLM_ERR_ARG, "path must be 0 (automatic), 1 (hop list) or 2 (on the fly)");
     if (W < -1) throw Error(FTLM_ERR_ARG, "tile width must be -1 (automatic), 0 (untiled) or positive");
+    if (W > (path == 2 ? kMaxTileOnTheFly : kMaxTileHopList))
+      throw Error(FTLM_ERR_LIMIT, path == 2 ? "on-the-fly column tiles hold at most 128 vectors"
+                                            : "hop-list column tiles hold at most 256 vectors");
     ctx->apply_path = path;
     ctx->apply_w = W;
     ctx->apply_path_r1 = path == 0 ? 2 : path;
@@ -1941,7 +1970,7 @@ int ftlm_autotune(ftlm_ctx* ctx, int R, int* best_path, int* best_w, double* bes
   return guarded([&] {
     if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
     need_block(ctx);
-    check_R(R);
+    check_batch_R(R);
     const int64_t n = ctx->cur.dim;
     if (n == 0) throw Error(FTLM_ERR_ARG, "empty block");
     HIPCHECK(hipSetDevice(ctx->device));

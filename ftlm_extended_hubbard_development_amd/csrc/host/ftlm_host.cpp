@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -106,6 +107,10 @@ void print_help(const char* argv0) {
             << "  --twist-grid N        run the twists phix, phiy in {0, 1/N, ..., (N-1)/N} together;\n"
             << "                        one CSV per twist, OUTPUT with .phixA_phiyB before the extension\n"
             << "  --twist-list L        like --twist-grid for the listed twists PX:PY,PX:PY,...\n"
+            << "  --partial-sums PATH   keep the grand-canonical sums of finished sectors in PATH and\n"
+            << "                        resume from it (a few KB; the CSV is written when all are done)\n"
+            << "  --stop-after S        stop between sectors before S seconds of wall time are exceeded\n"
+            << "                        (predicted from this run's rate); rerun the command to continue\n"
             << "  --help                show this message\n";
 }
 
@@ -162,6 +167,8 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
     else if (arg == "--pack-wg") p.pack_wg = parse_as<int>(next(arg));
     else if (arg == "--pack-budget-gb") p.pack_budget_gb = parse_as<double>(next(arg));
     else if (arg == "--twist-grid") p.twist_grid = parse_as<int>(next(arg));
+    else if (arg == "--partial-sums") p.partial_sums = next(arg);
+    else if (arg == "--stop-after") p.stop_after = parse_as<double>(next(arg));
     else if (arg == "--twist-list") {
       std::stringstream ss(next(arg));
       std::string item;
@@ -191,6 +198,10 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
   if (p.pack_wg < 0 || p.pack_budget_gb < 0.0) die("--pack-wg and --pack-budget-gb must be non-negative.");
   if (p.twist_grid < 0) die("--twist-grid must be non-negative.");
   if (p.twist_grid > 0 && !p.twist_list.empty()) die("--twist-grid and --twist-list are exclusive.");
+  if (p.stop_after < 0.0) die("--stop-after must be non-negative.");
+  if (p.stop_after > 0.0 && p.partial_sums.empty() && p.checkpoint.empty())
+    die("--stop-after needs --partial-sums or --checkpoint to resume from.");
+  if (!p.partial_sums.empty() && !p.checkpoint.empty()) die("--partial-sums and --checkpoint are exclusive.");
   const bool partial = p.dbg_nup >= 0 || p.dbg_ndn >= 0 || p.dbg_mx >= 0 || p.dbg_my >= 0 || !p.debug_csv.empty();
   const bool full = p.dbg_nup >= 0 && p.dbg_ndn >= 0 && p.dbg_mx >= 0 && p.dbg_my >= 0;
   if (partial && !full)
@@ -443,6 +454,110 @@ void write_multi_beta(const std::string& path, const std::vector<double>& betas,
           << r[b].z[i] << "," << r[b].logz[i] << "\n";
 }
 
+std::string partial_meta(const Options& o, const std::vector<std::pair<double, double>>& twists) {
+  std::ostringstream out;
+  out << metadata_text(o) << std::setprecision(17) << "betas=";
+  for (std::size_t i = 0; i < o.betas.size(); ++i) out << (i ? "," : "") << o.betas[i];
+  out << "\nmu=" << o.mu_min << "," << o.mu_max << "," << o.mu_count << "\nreduction="
+      << (o.reduction == 0 ? "serial" : "tree") << "\ntwists=";
+  for (std::size_t i = 0; i < twists.size(); ++i) out << (i ? ";" : "") << twists[i].first << ":" << twists[i].second;
+  out << "\n";
+  return out.str();
+}
+
+namespace {
+const char* kPartialMagic = "FTLM_PARTIAL_SUMS 1";
+
+std::string hexd(double v) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%a", v);
+  return b;
+}
+
+double parse_hexd(const std::string& t, const std::string& path) {
+  char* end = nullptr;
+  const double v = std::strtod(t.c_str(), &end);
+  if (end == t.c_str() || *end != '\0') die("Malformed partial-sum file: " + path);
+  return v;
+}
+}  // namespace
+
+bool read_partial(const std::string& path, PartialState& s) {
+  std::ifstream in(path);
+  if (!in) return false;
+  auto bad = [&]() { die("Malformed partial-sum file: " + path); };
+  std::string line, key;
+  if (!std::getline(in, line) || line != kPartialMagic) bad();
+  int nmeta = 0;
+  if (!(in >> key >> nmeta) || key != "meta_lines" || nmeta < 0) bad();
+  std::getline(in, line);
+  s.meta.clear();
+  for (int i = 0; i < nmeta; ++i) {
+    if (!std::getline(in, line)) bad();
+    s.meta += line + "\n";
+  }
+  std::string el;
+  if (!(in >> key >> s.next_nup >> s.next_ndn) || key != "next") bad();
+  if (!(in >> key >> s.runs) || key != "runs") bad();
+  if (!(in >> key >> el) || key != "elapsed") bad();
+  s.elapsed_s = parse_hexd(el, path);
+  if (!(in >> key >> s.blocks) || key != "blocks") bad();
+  if (!(in >> key >> s.ntwist >> s.nbeta >> s.nmu) || key != "shape") bad();
+  if (s.ntwist <= 0 || s.nbeta <= 0 || s.nmu <= 0) bad();
+  s.th.assign((std::size_t)s.ntwist * s.nbeta, Thermo((std::size_t)s.nmu));
+  for (Thermo& t : s.th)
+    for (int k = 0; k < s.nmu; ++k) {
+      std::string a, b, c, d;
+      if (!(in >> a >> b >> c >> d)) bad();
+      t.emin[k] = parse_hexd(a, path);
+      t.z[k] = parse_hexd(b, path);
+      t.n1[k] = parse_hexd(c, path);
+      t.n2[k] = parse_hexd(d, path);
+    }
+  if (!(in >> key) || key != "end") bad();
+  return true;
+}
+
+void write_partial(const std::string& path, const PartialState& s) {
+  const std::string tmp = path + ".tmp";
+  {
+    std::ofstream out(tmp);
+    if (!out) die("Failed to write partial-sum file: " + tmp);
+    int nmeta = 0;
+    for (char ch : s.meta) nmeta += ch == '\n';
+    out << kPartialMagic << "\nmeta_lines " << nmeta << "\n" << s.meta << "next " << s.next_nup << " "
+        << s.next_ndn << "\nruns " << s.runs << "\nelapsed " << hexd(s.elapsed_s) << "\nblocks " << s.blocks
+        << "\nshape " << s.ntwist << " " << s.nbeta << " " << s.nmu << "\n";
+    for (const Thermo& t : s.th)
+      for (int k = 0; k < s.nmu; ++k)
+        out << hexd(t.emin[k]) << " " << hexd(t.z[k]) << " " << hexd(t.n1[k]) << " " << hexd(t.n2[k]) << "\n";
+    out << "end\n";
+    out.flush();
+    if (!out) die("Failed to write partial-sum file: " + tmp);
+  }
+  if (std::rename(tmp.c_str(), path.c_str()) != 0) die("Failed to replace partial-sum file: " + path);
+}
+
+void merge_thermo(Thermo& run, const Thermo& part, double beta) {
+  for (std::size_t k = 0; k < run.z.size(); ++k) {
+    if (!(part.z[k] > 0.0)) continue;
+    if (!(run.z[k] > 0.0)) {
+      run.emin[k] = part.emin[k];
+      run.z[k] = part.z[k];
+      run.n1[k] = part.n1[k];
+      run.n2[k] = part.n2[k];
+      continue;
+    }
+    const double e = std::min(run.emin[k], part.emin[k]);
+    const double fr = std::exp(-beta * (run.emin[k] - e));
+    const double fp = std::exp(-beta * (part.emin[k] - e));
+    run.z[k] = run.z[k] * fr + part.z[k] * fp;
+    run.n1[k] = run.n1[k] * fr + part.n1[k] * fp;
+    run.n2[k] = run.n2[k] * fr + part.n2[k] * fp;
+    run.emin[k] = e;
+  }
+}
+
 }  // namespace ftlmh
 
 // ---- C hooks for the CPU tests of the host layer (no GPU involved) --------
@@ -586,6 +701,82 @@ int ftlmh_write_results(const char* path, int nbeta, const double* betas, int nm
     g_host_err = e.what();
     return 1;
   }
+}
+
+// Partial-sum files: sums[(t * nb + b) * 4 * nmu + q * nmu + k], q = emin, z,
+// n1, n2.
+int ftlmh_partial_write(const char* path, const char* meta, int next_nup, int next_ndn, int runs, double elapsed,
+                        long long blocks, int nt, int nb, int nmu, const double* sums) {
+  try {
+    ftlmh::PartialState s;
+    s.meta = meta;
+    s.next_nup = next_nup;
+    s.next_ndn = next_ndn;
+    s.runs = runs;
+    s.elapsed_s = elapsed;
+    s.blocks = blocks;
+    s.ntwist = nt;
+    s.nbeta = nb;
+    s.nmu = nmu;
+    s.th.assign((std::size_t)nt * nb, ftlmh::Thermo((std::size_t)nmu));
+    for (std::size_t i = 0; i < s.th.size(); ++i)
+      for (int k = 0; k < nmu; ++k) {
+        const double* q = sums + i * 4 * nmu;
+        s.th[i].emin[k] = q[k];
+        s.th[i].z[k] = q[nmu + k];
+        s.th[i].n1[k] = q[2 * nmu + k];
+        s.th[i].n2[k] = q[3 * nmu + k];
+      }
+    ftlmh::write_partial(path, s);
+    return 0;
+  } catch (const std::exception& e) {
+    g_host_err = e.what();
+    return -1;
+  }
+}
+
+// 1 read, 0 absent, -1 error; meta is NUL-terminated within cap bytes
+int ftlmh_partial_read(const char* path, char* meta, int cap, int* head, double* elapsed, long long* blocks,
+                       double* sums, long long cap_sums) {
+  try {
+    ftlmh::PartialState s;
+    if (!ftlmh::read_partial(path, s)) return 0;
+    if ((int)s.meta.size() >= cap) throw std::runtime_error("meta buffer too small");
+    std::memcpy(meta, s.meta.c_str(), s.meta.size() + 1);
+    head[0] = s.next_nup;
+    head[1] = s.next_ndn;
+    head[2] = s.runs;
+    head[3] = s.ntwist;
+    head[4] = s.nbeta;
+    head[5] = s.nmu;
+    *elapsed = s.elapsed_s;
+    *blocks = s.blocks;
+    if ((long long)s.th.size() * 4 * s.nmu > cap_sums) throw std::runtime_error("sums buffer too small");
+    for (std::size_t i = 0; i < s.th.size(); ++i)
+      for (int k = 0; k < s.nmu; ++k) {
+        double* q = sums + i * 4 * s.nmu;
+        q[k] = s.th[i].emin[k];
+        q[s.nmu + k] = s.th[i].z[k];
+        q[2 * s.nmu + k] = s.th[i].n1[k];
+        q[3 * s.nmu + k] = s.th[i].n2[k];
+      }
+    return 1;
+  } catch (const std::exception& e) {
+    g_host_err = e.what();
+    return -1;
+  }
+}
+
+// run, part: [4 * nmu] planes emin, z, n1, n2; run is updated in place
+void ftlmh_merge(int nmu, double beta, double* run, const double* part) {
+  ftlmh::Thermo a((std::size_t)nmu), b((std::size_t)nmu);
+  for (int k = 0; k < nmu; ++k) {
+    a.emin[k] = run[k], a.z[k] = run[nmu + k], a.n1[k] = run[2 * nmu + k], a.n2[k] = run[3 * nmu + k];
+    b.emin[k] = part[k], b.z[k] = part[nmu + k], b.n1[k] = part[2 * nmu + k], b.n2[k] = part[3 * nmu + k];
+  }
+  ftlmh::merge_thermo(a, b, beta);
+  for (int k = 0; k < nmu; ++k)
+    run[k] = a.emin[k], run[nmu + k] = a.z[k], run[2 * nmu + k] = a.n1[k], run[3 * nmu + k] = a.n2[k];
 }
 
 }  // extern "C"

@@ -43,6 +43,12 @@ struct Options {
   bool host_starts = false;
   // twist scan over listed (phix, phiy) pairs, one CSV per twist
   std::vector<std::pair<double, double>> twist_list;
+  // resumable runs: the grand-canonical sums of finished sectors kept in a
+  // small file (--partial-sums), and a wall-time budget after which the run
+  // stops between sectors (--stop-after, with --partial-sums or
+  // --checkpoint)
+  std::string partial_sums;
+  double stop_after = 0.0;
 };
 
 // parse_args (main.cpp:257-368). Returns false when --help was printed.
@@ -98,6 +104,28 @@ BlockThermo raw_thermo(const double* eig, const double* w, std::size_t cnt, doub
                        double beta, double mu);
 
 std::vector<double> linspace(double a, double b, int n);
+
+// Grand-canonical sums of the sectors finished so far, per (twist, beta)
+// stream, with the first sector still to run (--partial-sums). Written
+// atomically after every sector; doubles in hexadecimal, so a resumed run
+// continues from exactly the values an uninterrupted one would hold.
+struct PartialState {
+  std::string meta;          // identity of the run (refuses a different one)
+  int next_nup = 0, next_ndn = 0;
+  int runs = 0;              // processes that contributed
+  double elapsed_s = 0.0;    // their summed wall time
+  std::int64_t blocks = 0;   // blocks summed
+  int ntwist = 0, nbeta = 0, nmu = 0;
+  std::vector<Thermo> th;    // [twist * nbeta + beta]
+};
+std::string partial_meta(const Options& o, const std::vector<std::pair<double, double>>& twists);
+// false when the file does not exist; dies on a malformed file
+bool read_partial(const std::string& path, PartialState& s);
+void write_partial(const std::string& path, const PartialState& s);
+// fold the sums of a later set of blocks (same beta) into a running state:
+// both are rescaled to the smaller of their minima (main.cpp:1195-1216
+// applies the same rescale block by block)
+void merge_thermo(Thermo& run, const Thermo& part, double beta);
 
 struct Results {
   std::vector<double> n, cc, kappa, z, logz;

@@ -81,10 +81,10 @@ std::string twist_output(const std::string& out, double px, double py, int list_
   return out.substr(0, dot) + tag.str() + out.substr(dot);
 }
 
-// Thermodynamics of all stored spectra for every beta in one device launch
-// (ftlm_thermo), finalised as main.cpp:1770-1788.
-std::vector<Results> device_results(ftlm_ctx* c, const std::vector<Spectrum>& specs, const std::vector<double>& betas,
-                                    const std::vector<double>& mu, int sites) {
+// Grand-canonical sums of all stored spectra for every beta in one device
+// launch (ftlm_thermo): one Thermo per beta.
+std::vector<Thermo> device_sums(ftlm_ctx* c, const std::vector<Spectrum>& specs, const std::vector<double>& betas,
+                                const std::vector<double>& mu) {
   std::vector<std::int64_t> off(specs.size() + 1, 0);
   for (std::size_t b = 0; b < specs.size(); ++b) off[b + 1] = off[b] + (std::int64_t)specs[b].eig.size();
   std::vector<double> eig((std::size_t)off.back()), w((std::size_t)off.back(), 1.0), pref(specs.size());
@@ -101,7 +101,7 @@ std::vector<Results> device_results(ftlm_ctx* c, const std::vector<Spectrum>& sp
   check(ftlm_thermo(c, (std::int64_t)specs.size(), off.data(), eig.data(), w.data(), part.data(), pref.data(),
                     (int)betas.size(), betas.data(), (int)nmu, mu.data(), raw.data()),
         "thermodynamic sums");
-  std::vector<Results> all;
+  std::vector<Thermo> all;
   for (std::size_t i = 0; i < betas.size(); ++i) {
     Thermo th(nmu);
     for (std::size_t k = 0; k < nmu; ++k) {
@@ -111,14 +111,38 @@ std::vector<Results> device_results(ftlm_ctx* c, const std::vector<Spectrum>& sp
       th.n1[k] = r[2];
       th.n2[k] = r[3];
     }
-    all.push_back(finalize(th, betas[i], sites));
+    all.push_back(th);
   }
   return all;
+}
+
+// ... finalised as main.cpp:1770-1788.
+std::vector<Results> device_results(ftlm_ctx* c, const std::vector<Spectrum>& specs, const std::vector<double>& betas,
+                                    const std::vector<double>& mu, int sites) {
+  const std::vector<Thermo> th = device_sums(c, specs, betas, mu);
+  std::vector<Results> all;
+  for (std::size_t i = 0; i < betas.size(); ++i) all.push_back(finalize(th[i], betas[i], sites));
+  return all;
+}
+
+// Relative cost of a sector's FTLM blocks for the --stop-after prediction:
+// per block R * m * dim * (hop targets per row + 6 vector streams), the
+// algorithmic bytes of its Lanczos steps up to a constant factor.
+double sector_cost(const Options& o, int nup, int ndn, const std::vector<std::int64_t>& dims) {
+  const int L = o.lx * o.ly;
+  const double d = 2.0 * (o.tx != 0.0) + 2.0 * (o.ty != 0.0) + 4.0 * (o.tp != 0.0);
+  const double h = L > 1 ? d * ((double)nup * (L - nup) + (double)ndn * (L - ndn)) / (L - 1) : 0.0;
+  double c = 0.0;
+  for (std::int64_t dim : dims)
+    if ((std::size_t)dim > o.threshold)
+      c += (double)o.samples * (double)std::min<std::int64_t>(o.steps, dim) * (double)dim * (h + 6.0);
+  return c;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
+  const auto t_start = std::chrono::steady_clock::now();
   try {
     Options o;
     if (!parse_options(argc, argv, o, false)) return 0;
@@ -127,7 +151,10 @@ int main(int argc, char** argv) {
     // tree mode: keep every spectrum and sum them on the device at the end;
     // serial mode: the reference's streaming host accumulation (bit-identical)
     const bool device_thermo = o.reduction == FTLM_REDUCE_TREE;
-    const bool store = multi || device_thermo;
+    const bool partial = !o.partial_sums.empty();
+    // stored spectra: multi-beta, device sums, and resumable partial sums
+    // (folded into the running sums at the end of every sector)
+    const bool store = multi || device_thermo || partial;
     const std::vector<double> mu = linspace(o.mu_min, o.mu_max, o.mu_count);
     const bool trace = !o.trace_csv.empty();
     const bool dbg = o.dbg_nup >= 0 && o.dbg_ndn >= 0 && o.dbg_mx >= 0 && o.dbg_my >= 0;
@@ -136,6 +163,7 @@ int main(int argc, char** argv) {
     const bool scan = o.twist_grid > 0 || !o.twist_list.empty();
     if (scan && (!o.checkpoint.empty() || trace || dbg))
       die("--twist-grid/--twist-list are not compatible with --checkpoint or trace/debug block output.");
+    if (partial && (trace || dbg)) die("--partial-sums is not compatible with trace/debug block output.");
     std::vector<std::pair<double, double>> twists;
     if (o.twist_grid > 0) {
       for (int iy = 0; iy < o.twist_grid; ++iy)
@@ -147,6 +175,24 @@ int main(int argc, char** argv) {
       twists.push_back({o.phix, o.phiy});
     }
     const int ntw = (int)twists.size();
+    const int nbeta = (int)o.betas.size();
+    PartialState ps;
+    bool resumed = false;
+    if (partial) {
+      const std::string want = partial_meta(o, twists);
+      resumed = read_partial(o.partial_sums, ps);
+      if (resumed) {
+        if (ps.meta != want) die("Partial-sum file does not match current run: " + o.partial_sums);
+        if (ps.ntwist != ntw || ps.nbeta != nbeta || ps.nmu != (int)mu.size())
+          die("Partial-sum file does not match current run: " + o.partial_sums);
+      } else {
+        ps.meta = want;
+        ps.ntwist = ntw;
+        ps.nbeta = nbeta;
+        ps.nmu = (int)mu.size();
+        ps.th.assign((std::size_t)ntw * nbeta, Thermo(mu.size()));
+      }
+    }
     check_or_write_metadata(o);
     ensure_header(o.checkpoint);
     std::uint64_t valid = 0;
@@ -170,6 +216,9 @@ int main(int argc, char** argv) {
       std::cout << "twist list (" << ntw << " twists)\n";
     if (!o.checkpoint.empty())
       std::cout << "checkpoint=" << o.checkpoint << " loaded_blocks=" << ckpt.size() << "\n";
+    if (partial)
+      std::cout << "partial_sums=" << o.partial_sums << " summed_blocks=" << ps.blocks << " resume_at=Nup="
+                << (resumed ? ps.next_nup : 0) << ",Ndown=" << (resumed ? ps.next_ndn : 0) << "\n";
 
     Ctx ctx;
     const ftlm_model model{o.lx, o.ly, o.tx, o.ty, o.tp, twists[0].first, twists[0].second, o.u, o.v};
@@ -383,6 +432,37 @@ int main(int argc, char** argv) {
       drain();
     };
 
+    // ---- resumable runs: fold finished sectors into the running sums ----
+    const double base_elapsed = ps.elapsed_s;
+    const int base_runs = ps.runs;
+    auto seconds = [&]() { return ms_since(t_start) / 1000.0; };
+    auto fold = [&]() {
+      for (int t = 0; t < ntw; ++t) {
+        if (stored[t].empty()) continue;
+        if (device_thermo) {
+          const std::vector<Thermo> part = device_sums(ctx.c, stored[t], o.betas, mu);
+          for (int b = 0; b < nbeta; ++b) merge_thermo(ps.th[(std::size_t)t * nbeta + b], part[b], o.betas[b]);
+        } else {
+          // the reference's streaming sums, continued block by block
+          for (int b = 0; b < nbeta; ++b)
+            for (const Spectrum& sp : stored[t])
+              accumulate(sp.eig.data(), sp.w.empty() ? nullptr : sp.w.data(), sp.eig.size(), sp.prefactor,
+                         sp.particles, o.betas[b], mu, ps.th[(std::size_t)t * nbeta + b], threads);
+        }
+        ps.blocks += (std::int64_t)stored[t].size();
+        stored[t].clear();
+      }
+    };
+    auto save_partial = [&](int next_nup, int next_ndn) {
+      ps.next_nup = next_nup;
+      ps.next_ndn = next_ndn;
+      ps.runs = base_runs + 1;
+      ps.elapsed_s = base_elapsed + seconds();
+      write_partial(o.partial_sums, ps);
+    };
+    double work_here = 0.0;      // sector_cost of the sectors run by this process
+    double t_first_sector = -1.0;
+
     for (int nup = 0; nup <= L; ++nup) {
       for (int ndn = 0; ndn <= L; ++ndn) {
         std::uint64_t full = 0;
@@ -399,6 +479,37 @@ int main(int argc, char** argv) {
         std::vector<std::int64_t> dims(L);
         for (int my = 0; my < o.ly; ++my)
           for (int mx = 0; mx < o.lx; ++mx) check(ftlm_block(ctx.c, mx, my, &dims[my * o.lx + mx]), "block build");
+        const bool summed_before =
+            partial && resumed && (nup < ps.next_nup || (nup == ps.next_nup && ndn < ps.next_ndn));
+        if (!summed_before && o.stop_after > 0.0 && work_here > 0.0) {
+          const double el = seconds();
+          const double cost = sector_cost(o, nup, ndn, dims) * (double)ntw;
+          const double pred = cost * ((el - t_first_sector) / work_here);
+          if (el + pred > o.stop_after) {
+            if (partial) save_partial(nup, ndn);
+            if (tj) tj << "\n]\n";
+            std::cout << "Stopped before sector Nup=" << nup << " Ndown=" << ndn << " after " << el
+                      << " s (--stop-after " << o.stop_after << ", next sector predicted " << pred
+                      << " s); run the same command again to continue\n";
+            return 0;
+          }
+        }
+        if (summed_before) {
+          std::size_t active = 0;
+          for (std::int64_t d : dims) active += (std::size_t)d;
+          std::ostringstream line;
+          line << "  sector Nup=" << nup << " Ndown=" << ndn << " full_dim=" << full << " parents=" << parents
+               << " active_k_dim_sum=" << active << "\n";
+          Item li;
+          li.is_line = true;
+          li.line = line.str();
+          li.ready = true;
+          push(std::move(li));
+          drain();
+          continue;
+        }
+        if (t_first_sector < 0.0) t_first_sector = seconds();
+        work_here += sector_cost(o, nup, ndn, dims) * (double)ntw;
         std::size_t active = 0;
         for (int my = 0; my < o.ly; ++my) {
           for (int mx = 0; mx < o.lx; ++mx) {
@@ -484,9 +595,9 @@ int main(int argc, char** argv) {
             drain();
           }
         }
-        // a checkpointed run writes every sector's records before the next
-        // sector starts, so an interrupted run keeps its progress
-        if (!o.checkpoint.empty()) {
+        // a checkpointed (or partial-sum) run finishes every sector before
+        // the next one starts, so an interrupted run keeps its progress
+        if (!o.checkpoint.empty() || partial) {
           flush_pack();
           flush_exact();
         }
@@ -499,11 +610,22 @@ int main(int argc, char** argv) {
         li.ready = true;
         push(std::move(li));
         drain();
+        if (partial) {
+          if (!queue.empty()) die("internal error: unfinished blocks at the end of a sector");
+          fold();
+          save_partial(ndn < L ? nup : nup + 1, ndn < L ? ndn + 1 : 0);
+        }
       }
     }
     flush_pack();
     flush_exact();
     drain();
+    if (partial) {
+      fold();
+      save_partial(L + 1, 0);
+      std::cout << "partial sums: " << ps.blocks << " blocks summed over " << ps.runs << " run(s), "
+                << ps.elapsed_s << " s of wall time in total\n";
+    }
     if (!queue.empty()) die("internal error: unfinished blocks");
     if (tj) tj << "\n]\n";
     if (dbg && !o.debug_csv.empty() && !wrote_dbg)
@@ -514,7 +636,9 @@ int main(int argc, char** argv) {
           scan ? twist_output(o.output, twists[t].first, twists[t].second, o.twist_list.empty() ? -1 : t) : o.output;
       if (scan) std::cout << "twist phix=" << twists[t].first << " phiy=" << twists[t].second << "\n";
       std::vector<Results> all;
-      if (device_thermo) {
+      if (partial) {
+        for (int b = 0; b < nbeta; ++b) all.push_back(finalize(ps.th[(std::size_t)t * nbeta + b], o.betas[b], sites));
+      } else if (device_thermo) {
         all = device_results(ctx.c, stored[t], o.betas, mu, sites);
       } else if (multi) {
         for (double beta : o.betas) {

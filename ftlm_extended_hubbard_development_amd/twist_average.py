@@ -1,114 +1,150 @@
-"""Combine twist-resolved n(mu) CSV files (the consumer of twist scans).
+"""Combine twist-resolved n(mu) CSV files: the consumer of a twist scan
+(--twist-grid / --twist-list runs write one CSV per twist).
 
-Same command line and output as the reference's
-scripts/average_twist_outputs.py:8-116:
+Command line, messages and output columns are those of the reference's
+scripts/average_twist_outputs.py:40-116 (the drop-in contract):
 
     python -m ftlm_extended_hubbard_development_amd.twist_average twist_*.csv \
         --mode observable-average|partition-sum [--sites L] [--single-beta B] \
         --output out.csv
 
-Rows are grouped by (beta, mu). observable-average takes the plain mean of
-n, charge_correlation and compressibility over twists and the log of the
-mean partition function; partition-sum weights each twist by its partition
-function, w_t = Z_t / sum Z (computed in log space), and rebuilds n, the
-charge fluctuation and kappa from the mixed moments <N> and <N^2>.
+The implementation is a columnar one: every input row becomes one entry of
+flat numpy columns (beta, mu, n, charge_correlation, compressibility,
+log Z; NaN where a file has no log_partition), rows are grouped by
+(beta, mu) through np.unique, and each mode is a handful of segmented sums
+over the groups:
+
+* observable-average: n, charge_correlation and compressibility are the
+  plain means over the twists of a group; log_partition is the log of the
+  mean partition function, log(sum_t Z_t / T), over the rows that carry one
+  (empty when none does).
+* partition-sum: the twists form one ensemble, Z = sum_t Z_t, so twist t
+  enters with weight Z_t / Z (computed from log Z_t relative to the group's
+  largest); <N> and <N^2> are the weighted means of N_t = n_t L and
+  N2_t = cc_t L + N_t^2, n = <N> / L, cc = max(0, <N^2> - <N>^2) / L,
+  kappa = beta cc, log_partition = log Z.
 """
 import argparse
 import csv
-import math
-from collections import defaultdict
+
+import numpy as np
 
 FIELDS = ["beta", "mu", "n", "charge_correlation", "compressibility", "twist_count", "mode", "log_partition"]
+MODES = ("observable-average", "partition-sum")
 
 
-def log_sum_exp(values):
-    top = max(values)
-    return top + math.log(sum(math.exp(v - top) for v in values))
+class Table:
+    """Flat columns of all input rows, in input order."""
 
+    COLS = ("beta", "mu", "n", "charge_correlation", "compressibility", "log_partition")
 
-def read_rows(paths, single_beta=None):
-    """Rows of the input files; single-beta files (no beta column) need
-    single_beta (scripts/average_twist_outputs.py:15-38)."""
-    for path in paths:
+    def __init__(self):
+        self.cols = {c: [] for c in self.COLS}
+
+    def add_file(self, path, single_beta):
         with open(path, newline="") as f:
             reader = csv.DictReader(f)
-            has_beta = "beta" in (reader.fieldnames or [])
-            if not has_beta and single_beta is None:
+            names = reader.fieldnames or []
+            if "beta" not in names and single_beta is None:
                 raise SystemExit(f"{path} has no beta column; pass --single-beta for legacy single-beta files")
             for row in reader:
-                lp = row.get("log_partition")
-                yield {
-                    "beta": float(row["beta"]) if has_beta else float(single_beta),
-                    "mu": float(row["mu"]),
-                    "n": float(row["n"]),
-                    "charge_correlation": float(row["charge_correlation"]),
-                    "compressibility": float(row["compressibility"]),
-                    "log_partition": float(lp) if lp not in (None, "") else None,
-                }
+                self.cols["beta"].append(float(row["beta"]) if "beta" in names else float(single_beta))
+                for c in ("mu", "n", "charge_correlation", "compressibility"):
+                    self.cols[c].append(float(row[c]))
+                lz = row.get("log_partition")
+                self.cols["log_partition"].append(float(lz) if lz not in (None, "") else np.nan)
+
+    def arrays(self):
+        return {c: np.asarray(v, dtype=np.float64) for c, v in self.cols.items()}
 
 
-def combine(rows, mode, sites=None):
-    """One output record per (beta, mu), sorted; mode as on the command line
-    (scripts/average_twist_outputs.py:59-111)."""
+def load(paths, single_beta=None):
+    t = Table()
+    for p in paths:
+        t.add_file(p, single_beta)
+    return t.arrays()
+
+
+def _groups(a):
+    """(beta, mu) keys in ascending order, each row's group and the counts."""
+    keys, inv, cnt = np.unique(np.stack([a["beta"], a["mu"]], axis=1), axis=0, return_inverse=True,
+                               return_counts=True)
+    return keys, inv.ravel(), cnt
+
+
+def _segment_sum(values, inv, ngroups):
+    return np.bincount(inv, weights=values, minlength=ngroups)
+
+
+def _segment_max(values, inv, ngroups):
+    out = np.full(ngroups, -np.inf)
+    np.maximum.at(out, inv, values)
+    return out
+
+
+def combine(a, mode, sites=None):
+    """Records (dicts with FIELDS) per (beta, mu) in ascending order."""
+    if mode not in MODES:
+        raise SystemExit(f"unknown mode {mode}")
     if mode == "partition-sum" and (sites is None or sites <= 0):
         raise SystemExit("--sites is required and must be positive for --mode partition-sum")
-    groups = defaultdict(list)
-    for r in rows:
-        groups[(r["beta"], r["mu"])].append(r)
-    out = []
-    for (beta, mu), grp in sorted(groups.items()):
-        if mode == "observable-average":
-            k = len(grp)
-            n = sum(r["n"] for r in grp) / k
-            cc = sum(r["charge_correlation"] for r in grp) / k
-            kappa = sum(r["compressibility"] for r in grp) / k
-            logs = [r["log_partition"] for r in grp if r["log_partition"] is not None]
-            logz = log_sum_exp(logs) - math.log(len(logs)) if logs else ""
-        elif mode == "partition-sum":
-            if any(r["log_partition"] is None for r in grp):
-                raise SystemExit("partition-sum mode requires log_partition in every input row")
-            lz = [r["log_partition"] for r in grp]
-            total = log_sum_exp(lz)
-            wts = [math.exp(v - total) for v in lz]
-            m1 = 0.0
-            m2 = 0.0
-            for w, r in zip(wts, grp):
-                nt = r["n"] * sites
-                n2t = r["charge_correlation"] * sites + nt * nt
-                m1 += w * nt
-                m2 += w * n2t
-            n = m1 / sites
-            cc = max(0.0, (m2 - m1 * m1) / sites)
-            kappa = beta * cc
-            logz = total
-        else:
-            raise SystemExit(f"unknown mode {mode}")
-        out.append({"beta": beta, "mu": mu, "n": n, "charge_correlation": cc, "compressibility": kappa,
-                    "twist_count": len(grp), "mode": mode, "log_partition": logz})
-    return out
+    keys, inv, cnt = _groups(a)
+    G = len(keys)
+    lz = a["log_partition"]
+    have = ~np.isnan(lz)
+    if mode == "partition-sum" and not have.all():
+        raise SystemExit("partition-sum mode requires log_partition in every input row")
+    # log sum_t Z_t over the rows carrying log Z, relative to the group's top
+    top = _segment_max(np.where(have, lz, -np.inf), inv, G)
+    nz = _segment_sum(have.astype(np.float64), inv, G)
+    safe_top = np.where(np.isfinite(top), top, 0.0)
+    rel = np.where(have, np.exp(np.where(have, lz, 0.0) - safe_top[inv]), 0.0)
+    log_total = safe_top + np.log(np.where(nz > 0, _segment_sum(rel, inv, G), 1.0))
+    if mode == "observable-average":
+        n = _segment_sum(a["n"], inv, G) / cnt
+        cc = _segment_sum(a["charge_correlation"], inv, G) / cnt
+        kappa = _segment_sum(a["compressibility"], inv, G) / cnt
+        logz = [float(v) if k > 0 else "" for v, k in zip(log_total - np.log(np.maximum(nz, 1.0)), nz)]
+    else:
+        L = float(sites)
+        w = np.exp(lz - log_total[inv])
+        N1 = a["n"] * L
+        N2 = a["charge_correlation"] * L + N1 * N1
+        m1 = _segment_sum(w * N1, inv, G)
+        m2 = _segment_sum(w * N2, inv, G)
+        n = m1 / L
+        cc = np.maximum(0.0, (m2 - m1 * m1) / L)
+        kappa = keys[:, 0] * cc
+        logz = [float(v) for v in log_total]
+    return [{"beta": float(keys[g, 0]), "mu": float(keys[g, 1]), "n": float(n[g]), "charge_correlation": float(cc[g]),
+             "compressibility": float(kappa[g]), "twist_count": int(cnt[g]), "mode": mode, "log_partition": logz[g]}
+            for g in range(G)]
 
 
 def write(path, records):
     with open(path, "w", newline="") as f:
-        w = csv.DictWriter(f, fieldnames=FIELDS)
-        w.writeheader()
-        for rec in records:
-            w.writerow(rec)
+        out = csv.writer(f)
+        out.writerow(FIELDS)
+        for r in records:
+            out.writerow([r[k] for k in FIELDS])
 
 
-def main(argv=None):
+def parser():
     p = argparse.ArgumentParser(description="Average or partition-sum FTLM n(mu) CSV files over twists.")
     p.add_argument("inputs", nargs="+", help="twist-resolved CSV files")
     p.add_argument("--output", required=True, help="output CSV path")
-    p.add_argument("--mode", choices=["observable-average", "partition-sum"], default="observable-average",
+    p.add_argument("--mode", choices=list(MODES), default="observable-average",
                    help="average observables equally, or combine twists by summing partition functions")
     p.add_argument("--sites", type=int, help="number of lattice sites; required for partition-sum")
-    p.add_argument("--single-beta", type=float,
-                   help="beta value for legacy single-beta files without a beta column")
-    a = p.parse_args(argv)
-    if a.mode == "partition-sum" and (a.sites is None or a.sites <= 0):
+    p.add_argument("--single-beta", type=float, help="beta value for legacy single-beta files without a beta column")
+    return p
+
+
+def main(argv=None):
+    args = parser().parse_args(argv)
+    if args.mode == "partition-sum" and (args.sites is None or args.sites <= 0):
         raise SystemExit("--sites is required and must be positive for --mode partition-sum")
-    write(a.output, combine(read_rows(a.inputs, a.single_beta), a.mode, a.sites))
+    write(args.output, combine(load(args.inputs, args.single_beta), args.mode, args.sites))
 
 
 if __name__ == "__main__":

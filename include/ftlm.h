@@ -50,7 +50,8 @@ typedef struct ftlm_model {
   double tx, ty, tp, phix, phiy, u, v;
 } ftlm_model;
 
-/* Device-event timings of the last Lanczos run on a context. */
+/* Device-event timings of the last Lanczos run on a context (ftlm_run, or
+ * the batched runs of the last ftlm_ftlm_block call). */
 typedef struct ftlm_stats {
   double ms_apply;     /* summed H-apply kernel time */
   double ms_total;     /* first to last kernel of the run */
@@ -125,7 +126,11 @@ int ftlm_ritz(ftlm_ctx* ctx, int R, int steps, const double* alpha, const double
               int wdiv, int method, double* eig, double* w);
 
 /* run_ftlm_block: R seeded samples, `steps` Lanczos steps, Ritz values and
- * weights concatenated in sample order (eig/w hold R*min(steps, dim)). */
+ * weights concatenated in sample order (eig/w hold R*min(steps, dim)).
+ * R <= 1024; R above 256 runs in batches of 256 samples (one batched launch
+ * holds at most 256 columns; ftlm_apply, ftlm_lanczos without per_vector,
+ * ftlm_prepare and ftlm_autotune take at most 256 and return FTLM_ERR_LIMIT
+ * beyond). */
 int ftlm_ftlm_block(ftlm_ctx* ctx, int R, int steps, uint64_t block_seed, int reduce, int per_vector,
                     double* eig, double* w, int64_t* count);
 
@@ -142,7 +147,8 @@ int ftlm_last_stats(ftlm_ctx* ctx, ftlm_stats* out);
  * coefficient code, 8 bytes each) are built once on the device and every
  * apply is a pure gather -- or 2 on the fly -- hops are recomputed from the
  * basis tables inside each apply. W: -1 automatic, 0 untiled, W > 0 column
- * tiles of W vectors. ftlm_get_apply_config reports what R vectors on the
+ * tiles of W vectors, at most 256 (hop list) or 128 (on the fly; larger W
+ * returns FTLM_ERR_LIMIT). ftlm_get_apply_config reports what R vectors on the
  * current block use. */
 int ftlm_set_apply_config(ftlm_ctx* ctx, int path, int W);
 int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W);

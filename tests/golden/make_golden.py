@@ -6,11 +6,14 @@ here comes from the reference's own functions (build_lattice,
 build_sector_basis, build_momentum_block, apply_hamiltonian, run_ftlm_block,
 jacobi_diagonalize, accumulate_block_thermo, write_results, the checkpoint
 writer) except the eigenvalues of exactly diagonalised blocks: the reference
-calls LAPACK zheev there, which is not linked into the harness, so those come
-from numpy.linalg.eigvalsh of the reference's own Hermitised dense block
-(main.cpp:862-886) -- a library Hermitian eigensolver like zheev.
+calls LAPACK zheev('N', 'U') there (main.cpp:888-910), which the harness does
+not link, so this script calls the same LAPACK routine itself -- zheev of
+Intel MKL (the LAPACK the reference was built against when it was measured,
+loaded from /opt/conda/lib/libmkl_rt.so through ctypes, sequential layer) --
+on the reference's own Hermitised dense block (main.cpp:862-886).
 
     python tests/golden/make_golden.py            # writes the fixtures
+    python tests/golden/make_golden.py zheev csv  # only the exact-block ones
 """
 import json
 import math
@@ -29,8 +32,46 @@ if ref is None:
 
 
 def hermitised(blk):
+    """main.cpp:875-886: the diagonal's imaginary part dropped, (i, j) and
+    conj(j, i) averaged (0.5 * (a + conj(b)) rounds like numpy's expression)."""
     H = blk.dense()
-    return 0.5 * (H + H.conj().T)
+    A = 0.5 * (H + H.conj().T)
+    A[np.diag_indices_from(A)] = H.diagonal().real
+    return A
+
+
+ZHEEV_SOURCE = "LAPACK zheev('N','U') of Intel MKL (libmkl_rt, sequential)"
+_mkl = None
+
+
+def zheev(A):
+    """Eigenvalues (ascending) by LAPACK zheev with jobz='N', uplo='U', the
+    call of main.cpp:888-910 (workspace query, then the solve)."""
+    global _mkl
+    import ctypes as C
+
+    if _mkl is None:
+        os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
+        _mkl = C.CDLL("/opt/conda/lib/libmkl_rt.so")
+    n = A.shape[0]
+    a = np.asfortranarray(A.astype(np.complex128))
+    w = np.zeros(n)
+    rwork = np.zeros(max(1, 3 * n - 2))
+    info = C.c_int(0)
+    jobz, uplo, nn = C.c_char(b"N"), C.c_char(b"U"), C.c_int(n)
+    dp = C.POINTER(C.c_double)
+    q = np.zeros(2)
+    lwork = C.c_int(-1)
+    _mkl.zheev_(C.byref(jobz), C.byref(uplo), C.byref(nn), a.ctypes.data_as(dp), C.byref(nn), w.ctypes.data_as(dp),
+                q.ctypes.data_as(dp), C.byref(lwork), rwork.ctypes.data_as(dp), C.byref(info))
+    assert info.value == 0
+    lw = max(1, int(q[0]))
+    work = np.zeros(2 * lw)
+    lwork = C.c_int(lw)
+    _mkl.zheev_(C.byref(jobz), C.byref(uplo), C.byref(nn), a.ctypes.data_as(dp), C.byref(nn), w.ctypes.data_as(dp),
+                work.ctypes.data_as(dp), C.byref(lwork), rwork.ctypes.data_as(dp), C.byref(info))
+    assert info.value == 0
+    return w
 
 
 def exact_eigs(blk):
@@ -41,7 +82,7 @@ def exact_eigs(blk):
             if col[e] == 0:
                 ev += val[e].real
         return np.array([ev])
-    return np.linalg.eigvalsh(hermitised(blk))
+    return zheev(hermitised(blk))
 
 
 def sweep(model, *, beta_list, mu_min, mu_max, mu_count, samples, steps, threshold=256, seed=12345):
@@ -123,6 +164,18 @@ CSV_CASES = {
     "ed_4x2_beta20": (dict(lx=4, ly=2, u=8.0, v=0.0, tp=0.0),
                       dict(beta_list=[20.0], mu_min=-5.0, mu_max=15.0, mu_count=81, samples=32, steps=120,
                            threshold=1000000000)),
+    # the 3x4 model of the twist-averaged scan (t' = -0.25, U = 6, V = 1,
+    # beta = 10) at two twists of its 4x4 grid, few samples
+    "cfg4_3x4_tw_0.25_0.75_r4_s40": (dict(lx=3, ly=4, tp=-0.25, u=6.0, v=1.0, phix=0.25, phiy=0.75),
+                                     dict(beta_list=[10.0], mu_min=-2.0, mu_max=8.0, mu_count=21, samples=4,
+                                          steps=40)),
+    "cfg4_3x4_tw_0.5_0_r4_s40": (dict(lx=3, ly=4, tp=-0.25, u=6.0, v=1.0, phix=0.5, phiy=0.0),
+                                 dict(beta_list=[10.0], mu_min=-2.0, mu_max=8.0, mu_count=21, samples=4,
+                                      steps=40)),
+    # every block exact on the twisted 3x3 lattice (blocks up to 1,764 rows)
+    "ed_3x3_twist_beta10": (dict(lx=3, ly=3, tp=-0.25, u=6.0, v=1.0, phix=0.25, phiy=0.5),
+                            dict(beta_list=[10.0], mu_min=-5.0, mu_max=15.0, mu_count=41, samples=4, steps=60,
+                                 threshold=1000000000)),
 }
 
 
@@ -157,7 +210,7 @@ def csv_fixtures(only=None):
         mu, states, L = sweep(model, **kw)
         write_csv(os.path.join(HERE, "csv", name + ".csv"), kw["beta_list"], mu, states, L)
         meta[name] = {"args": command_line(model, kw),
-                      "exact_blocks_from": "numpy.linalg.eigvalsh" if kw.get("threshold", 256) > 0 else "none"}
+                      "exact_blocks_from": ZHEEV_SOURCE if kw.get("threshold", 256) > 0 else "none"}
     with open(os.path.join(HERE, "csv", "cases.json"), "w") as f:
         json.dump(meta, f, indent=1)
 
@@ -244,6 +297,36 @@ def block_fixtures():
         json.dump(index, f, indent=1)
 
 
+ZHEEV_CASES = [  # (model, blocks): exact-block eigenvalues at every size the engine's solver meets
+    (dict(lx=2, ly=2, tp=0.2, u=4.0, v=1.0), [(1, 1, 0, 0), (2, 1, 1, 0), (2, 2, 1, 1)]),
+    (dict(lx=4, ly=2, u=8.0), [(4, 4, 0, 0), (3, 4, 1, 1), (2, 3, 2, 0), (1, 1, 0, 0)]),
+    (dict(lx=3, ly=3, tp=-0.25, u=6.0, v=1.0, phix=0.25, phiy=0.5), [(4, 5, 0, 0), (4, 4, 1, 2), (3, 3, 2, 1),
+                                                                     (2, 1, 1, 1)]),
+    (dict(lx=4, ly=4, u=7.0), [(1, 2, 0, 0), (2, 1, 3, 1), (15, 15, 0, 0)]),
+    (dict(lx=3, ly=4, tp=-0.25, u=6.0, v=1.0, phix=0.25, phiy=0.75), [(2, 1, 1, 2), (11, 10, 0, 3)]),
+    (dict(lx=4, ly=2, u=0.0, v=0.0, phix=0.25, phiy=0.5), [(4, 4, 1, 0)]),
+]
+
+
+def zheev_fixtures():
+    """Eigenvalues of exact blocks by the reference's LAPACK call (zheev on
+    its Hermitised dense block), keyed zheev_m<model>_<nup>_<ndn>_<mx>_<my>."""
+    data = {}
+    index = []
+    for mi, (model, blocks) in enumerate(ZHEEV_CASES):
+        m = oracle_lib.model(**model)
+        for (nup, ndn, mx, my) in blocks:
+            blk = ref.block(m, nup, ndn, mx, my)
+            key = f"zheev_m{mi}_{nup}_{ndn}_{mx}_{my}"
+            print(key, blk.dim, flush=True)
+            data[key] = exact_eigs(blk)
+            index.append(dict(key=key, model=model, block=[nup, ndn, mx, my], dim=int(blk.dim),
+                              source=ZHEEV_SOURCE))
+    np.savez_compressed(os.path.join(HERE, "zheev.npz"), **data)
+    with open(os.path.join(HERE, "zheev.json"), "w") as f:
+        json.dump(index, f, indent=1)
+
+
 def primitive_fixtures():
     out = {"lattices": {}, "twist": [], "mix": [], "block_seed": [], "hop_sign": [], "diag": [], "translate": [],
            "combos": {}}
@@ -324,13 +407,15 @@ def checkpoint_fixture():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["primitives", "blocks", "checkpoint", "csv"]
+    which = sys.argv[1:] or ["primitives", "blocks", "checkpoint", "zheev", "csv"]
     if "primitives" in which:
         primitive_fixtures()
     if "blocks" in which:
         block_fixtures()
     if "checkpoint" in which:
         checkpoint_fixture()
+    if "zheev" in which:
+        zheev_fixtures()
     if "csv" in which:
         csv_fixtures()
     extra = [w for w in which if w in CSV_CASES]

@@ -302,3 +302,48 @@ def test_exact_blocks(pkg, orc, name, nup, ndn):
         assert np.max(np.abs(ev - ref)) < 1e-12 * scale
         if blk.dim == 1:
             assert ev[0] == orc.exact_block(blk)[0]
+
+
+@pytest.mark.parametrize("R", [257, 300, 512])
+@pytest.mark.parametrize("reduce", ["serial", "tree"])
+def test_ftlm_block_beyond_one_batch(pkg, orc, R, reduce):
+    """More samples than one batched launch holds (256 columns): the block
+    runs in batches of samples and gives the reference's spectrum, bit for
+    bit with serial reductions, with weights divided by the full R."""
+    case = ("4x2", 3, 4, 1, 1)
+    eng, blk = both(pkg, orc, *case)
+    bs = pkg.block_seed(12345, *case[1:])
+    e, w = eng.ftlm_block(R, 40, bs, reduce=pkg.REDUCE_SERIAL if reduce == "serial" else pkg.REDUCE_TREE)
+    oe, ow = orc.ftlm_block(blk, R, 40, bs)
+    assert len(e) == len(oe)
+    if reduce == "serial":
+        assert np.array_equal(e, oe) and np.array_equal(w, ow)
+    else:
+        assert abs(w.sum() - 1.0) < 1e-12
+        for mu in np.linspace(-3, 10, 7):
+            z, m = _block_n(e, w, blk.dim, 7, 20.0, mu)
+            zo, mo = _block_n(oe, ow, blk.dim, 7, 20.0, mu)
+            assert abs(m - mo) < 1e-10 and abs(z - zo) <= 1e-10 * abs(zo)
+
+
+def test_batched_calls_reject_more_than_256_columns(pkg, orc):
+    """The single-launch entry points hold at most 256 columns and say so
+    (FTLM_ERR_LIMIT) instead of leaving columns unwritten; tile widths are
+    bounded per apply path."""
+    eng, blk = both(pkg, orc, "4x2", 3, 4, 1, 1)
+    x = np.zeros((blk.dim, 257), np.complex128)
+    with pytest.raises(pkg.FtlmError, match="error -5"):
+        eng.apply(x)
+    with pytest.raises(pkg.FtlmError, match="error -5"):
+        eng.lanczos(257, 10, bseed=1)
+    with pytest.raises(pkg.FtlmError, match="error -5"):
+        eng.prepare(300, 1)
+    with pytest.raises(pkg.FtlmError, match="error -5"):
+        eng.set_apply_config(2, 256)
+    with pytest.raises(pkg.FtlmError, match="error -5"):
+        eng.set_apply_config(1, 512)
+    eng.set_apply_config(1, 256)
+    eng.set_apply_config(0, -1)
+    # per-vector runs take any R up to 1024
+    a, b, m = eng.lanczos(257, 5, bseed=1, reduce=pkg.REDUCE_SERIAL, per_vector=True)
+    assert np.all(m == 5)

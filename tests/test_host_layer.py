@@ -41,6 +41,11 @@ def host():
     L.ftlmh_accumulate.argtypes = [dp, dp, C.c_longlong, C.c_double, C.c_int, C.c_double, dp, C.c_int, dp, dp, dp,
                                    dp, C.c_int]
     L.ftlmh_write_results.argtypes = [C.c_char_p, C.c_int, dp, C.c_int, dp, dp, dp, dp, dp, dp]
+    L.ftlmh_partial_write.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_longlong,
+                                      C.c_int, C.c_int, C.c_int, dp]
+    L.ftlmh_partial_read.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int), dp,
+                                     C.POINTER(C.c_longlong), dp, C.c_longlong]
+    L.ftlmh_merge.argtypes = [C.c_int, C.c_double, dp, dp]
     return L
 
 
@@ -69,6 +74,21 @@ BAD = [
 
 @pytest.mark.parametrize("args,msg", BAD, ids=[" ".join(a) for a, _ in BAD])
 def test_parse_errors(host, args, msg):
+    n, a = argv(args)
+    assert host.ftlmh_parse(n, a) == 1
+    assert host.ftlmh_last_error().decode() == msg
+
+
+# the engine's additive flags for resumable runs
+BAD_ENGINE = [
+    (["--stop-after", "10"], "--stop-after needs --partial-sums or --checkpoint to resume from."),
+    (["--stop-after", "-1", "--partial-sums", "p"], "--stop-after must be non-negative."),
+    (["--partial-sums", "p", "--checkpoint", "c"], "--partial-sums and --checkpoint are exclusive."),
+]
+
+
+@pytest.mark.parametrize("args,msg", BAD_ENGINE, ids=[" ".join(a) for a, _ in BAD_ENGINE])
+def test_parse_errors_engine_flags(host, args, msg):
     n, a = argv(args)
     assert host.ftlmh_parse(n, a) == 1
     assert host.ftlmh_last_error().decode() == msg
@@ -228,3 +248,68 @@ def test_write_results_bytes(host, tmp_path):
         assert host.ftlmh_write_results(out.encode(), len(betas) if multi else 1, b.ctypes.data_as(dp), nmu,
                                         mu.ctypes.data_as(dp), *[c.ctypes.data_as(dp) for c in cols]) == 0
         assert open(out, "rb").read() == open(gold, "rb").read()
+
+
+def test_partial_sum_file_roundtrip(host, tmp_path):
+    """--partial-sums files keep every double exactly (hexadecimal), the
+    infinite minima of empty streams included, and the run identity."""
+    path = str(tmp_path / "p.sums").encode()
+    rng = np.random.default_rng(5)
+    nt, nb, nmu = 2, 3, 7
+    sums = rng.standard_normal(nt * nb * 4 * nmu) * 10.0 ** rng.integers(-300, 300, nt * nb * 4 * nmu)
+    sums[0] = np.inf
+    sums[5] = 0.0
+    meta = b"lx=4\nly=4\nbetas=2.857142857142857,12.5\n"
+    assert host.ftlmh_partial_write(path, meta, 5, 11, 3, 1234.5678, 999, nt, nb, nmu, sums.ctypes.data_as(dp)) == 0
+    buf = C.create_string_buffer(4096)
+    head = (C.c_int * 6)()
+    el = C.c_double(0)
+    nblk = C.c_longlong(0)
+    back = np.zeros_like(sums)
+    assert host.ftlmh_partial_read(path, buf, 4096, head, C.byref(el), C.byref(nblk), back.ctypes.data_as(dp),
+                                   back.size) == 1
+    assert buf.value == meta and list(head) == [5, 11, 3, nt, nb, nmu]
+    assert el.value == 1234.5678 and nblk.value == 999
+    assert np.array_equal(back.view(np.uint64), sums.view(np.uint64))
+    assert host.ftlmh_partial_read(str(tmp_path / "absent").encode(), buf, 4096, head, C.byref(el), C.byref(nblk),
+                                   back.ctypes.data_as(dp), back.size) == 0
+    (tmp_path / "bad.sums").write_text("FTLM_PARTIAL_SUMS 1\nmeta_lines 1\nx\nnext 1\n")
+    assert host.ftlmh_partial_read(str(tmp_path / "bad.sums").encode(), buf, 4096, head, C.byref(el),
+                                   C.byref(nblk), back.ctypes.data_as(dp), back.size) == -1
+    assert b"Malformed partial-sum file" in host.ftlmh_last_error()
+
+
+def test_merge_equals_one_stream(host, orc):
+    """Folding the sums of a second set of blocks into those of a first
+    gives the sums of one stream over all of them (the reference's streaming
+    accumulation, main.cpp:1172-1219) to rounding; an empty side is the
+    identity."""
+    rng = np.random.default_rng(9)
+    mu = orc.linspace(-3.0, 4.0, 41)
+    beta = 12.5
+    blocks = []
+    for _ in range(12):
+        n = int(rng.integers(1, 50))
+        e = np.sort(rng.standard_normal(n) * 5 - 20)
+        w = rng.random(n)
+        w /= w.sum()
+        blocks.append((e, w, float(rng.integers(1, 500)), int(rng.integers(0, 17))))
+
+    def stream(bl):
+        st = dict(emin=np.full(41, np.inf), z=np.zeros(41), n1=np.zeros(41), n2=np.zeros(41))
+        for e, w, pref, N in bl:
+            orc.accumulate(e, w, pref, N, beta, mu, st)
+        return np.concatenate([st["emin"], st["z"], st["n1"], st["n2"]])
+
+    whole = stream(blocks)
+    run = stream(blocks[:5])
+    part = stream(blocks[5:])
+    host.ftlmh_merge(41, beta, run.ctypes.data_as(dp), part.ctypes.data_as(dp))
+    assert np.array_equal(run[:41], whole[:41])
+    assert np.allclose(run[41:], whole[41:], rtol=1e-13, atol=0)
+    empty = np.concatenate([np.full(41, np.inf), np.zeros(123)])
+    before = part.copy()
+    host.ftlmh_merge(41, beta, part.ctypes.data_as(dp), empty.ctypes.data_as(dp))
+    assert np.array_equal(part, before)
+    host.ftlmh_merge(41, beta, empty.ctypes.data_as(dp), before.ctypes.data_as(dp))
+    assert np.array_equal(empty, before)

@@ -1,4 +1,4 @@
-"""Twist averaging (restatement of scripts/average_twist_outputs.py) pinned
+"""Twist averaging (the contract of scripts/average_twist_outputs.py) pinned
 by hand-computed cases: for two twists with known Z_t, <N>_t, <N^2>_t the
 partition-sum result must equal the statistics of the merged ensemble, and
 observable-average the plain means."""
@@ -39,7 +39,7 @@ def test_partition_sum_equals_merged_ensemble(tmp_path):
         p = tmp_path / f"t{t}.csv"
         write_csv(p, rows)
         files.append(str(p))
-    out = ta.combine(ta.read_rows(files), "partition-sum", L)
+    out = ta.combine(ta.load(files), "partition-sum", L)
     for rec, mu in zip(out, (-1.0, 0.0, 0.75)):
         merged = twists[0] + twists[1]
         _, z, n1, n2 = ensemble(L, beta, mu, merged)
@@ -58,8 +58,8 @@ def test_observable_average(tmp_path):
     write_csv(tmp_path / "a.csv", a, with_beta=False)
     write_csv(tmp_path / "b.csv", b, with_beta=False)
     with pytest.raises(SystemExit):
-        list(ta.read_rows([str(tmp_path / "a.csv")]))
-    out = ta.combine(ta.read_rows([str(tmp_path / "a.csv"), str(tmp_path / "b.csv")], single_beta=2.0),
+        ta.load([str(tmp_path / "a.csv")])
+    out = ta.combine(ta.load([str(tmp_path / "a.csv"), str(tmp_path / "b.csv")], single_beta=2.0),
                      "observable-average")
     (r,) = out
     assert (r["beta"], r["mu"], r["twist_count"]) == (2.0, 0.0, 2)
@@ -76,3 +76,29 @@ def test_cli_and_sites_guard(tmp_path):
     text = (tmp_path / "o.csv").read_text().splitlines()
     assert text[0] == "beta,mu,n,charge_correlation,compressibility,twist_count,mode,log_partition"
     assert text[1].startswith("1.0,0.0,") and ",1,partition-sum," in text[1]
+
+
+def test_groups_sorted_and_missing_log_partition(tmp_path):
+    """Rows of several files and betas group by (beta, mu) in ascending
+    order; observable-average leaves log_partition empty for a group with
+    no log Z and averages the ones present; partition-sum refuses a file
+    without log Z."""
+    a = [{"beta": 2.0, "mu": 1.0, "n": 0.5, "charge_correlation": 0.1, "compressibility": 0.2,
+          "partition_like": 1, "log_partition": ""},
+         {"beta": 1.0, "mu": 1.0, "n": 0.2, "charge_correlation": 0.1, "compressibility": 0.1,
+          "partition_like": 1, "log_partition": 0.5}]
+    b = [{"beta": 1.0, "mu": 1.0, "n": 0.4, "charge_correlation": 0.3, "compressibility": 0.3,
+          "partition_like": 1, "log_partition": 1.5},
+         {"beta": 2.0, "mu": 1.0, "n": 0.7, "charge_correlation": 0.3, "compressibility": 0.6,
+          "partition_like": 1, "log_partition": ""},
+         {"beta": 1.0, "mu": -1.0, "n": 0.1, "charge_correlation": 0.0, "compressibility": 0.0,
+          "partition_like": 1, "log_partition": 2.0}]
+    write_csv(tmp_path / "a.csv", a)
+    write_csv(tmp_path / "b.csv", b)
+    out = ta.combine(ta.load([str(tmp_path / "a.csv"), str(tmp_path / "b.csv")]), "observable-average")
+    assert [(r["beta"], r["mu"], r["twist_count"]) for r in out] == [(1.0, -1.0, 1), (1.0, 1.0, 2), (2.0, 1.0, 2)]
+    assert abs(out[1]["n"] - 0.3) < 1e-15 and out[2]["log_partition"] == ""
+    assert abs(out[0]["log_partition"] - 2.0) < 1e-15
+    assert abs(out[1]["log_partition"] - math.log((math.exp(0.5) + math.exp(1.5)) / 2)) < 1e-14
+    with pytest.raises(SystemExit):
+        ta.combine(ta.load([str(tmp_path / "a.csv")]), "partition-sum", 4)
