@@ -234,11 +234,7 @@ __global__ void __launch_bounds__(kThreads) k_apply(const LatticeConst* __restri
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int r = tid; r < R; r += kThreads) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
-    dot_out[r] = s;
-  }
+  last_block_column_sums(partials, gridDim.x, R, [&](int r, double sum) { dot_out[r] = sum; });
   if (tid == 0) *counter = 0u;
 }
 
@@ -435,11 +431,7 @@ __global__ void __launch_bounds__(kThreads) k_apply_tiled(const LatticeConst* __
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int r = tid; r < R; r += kThreads) {
-    double sum = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) sum += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
-    dot_out[r] = sum;
-  }
+  last_block_column_sums(partials, gridDim.x, R, [&](int r, double sum) { dot_out[r] = sum; });
   if (tid == 0) *counter = 0u;
 }
 
@@ -503,12 +495,8 @@ __global__ void __launch_bounds__(kThreads) k_apply_r1(const LatticeConst* __res
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[b]);
-    dot_out[0] = s;
-    *counter = 0u;
-  }
+  last_block_column_sums(partials, gridDim.x, 1, [&](int, double sum) { dot_out[0] = sum; });
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // ---- hop lists: the block's hops built once, reused by every apply -------
@@ -777,12 +765,69 @@ k_apply_hl(int64_t dim, int R, int W, int J, const int64_t* __restrict__ offs, i
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int r = threadIdx.x; r < R; r += 256) {
-    double sum = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) sum += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
-    dot_out[r] = sum;
-  }
+  last_block_column_sums(partials, gridDim.x, R, [&](int r, double sum) { dot_out[r] = sum; });
   if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Column-tiled gather (see TileTask): lane l of a wavefront owns column
+// tile * W + l % W of row row0 + wave * (64 / W) + l / W; each lane adds its
+// row's hops in list order after the diagonal term (the reference's order),
+// four at a time, while any row of the wavefront has hops left.
+template <int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+k_apply_hl_tiles(int R, int64_t ntask, const TileTask* __restrict__ tasks, const int64_t* __restrict__ row_ptr,
+                 const int32_t* __restrict__ hcol, const uint32_t* __restrict__ hcode,
+                 const double2* __restrict__ ptab, const double2* __restrict__ spec, const double* __restrict__ diag,
+                 const double2* __restrict__ x, double2* __restrict__ y) {
+  constexpr int RPW = 64 / W;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int q = lane / W, w = lane % W;
+  int64_t t1 = ntask, tstep = gridDim.x, t = blockIdx.x;
+  if (gridDim.x >= kXcd) {
+    const int xcd = blockIdx.x % kXcd;
+    const int l = blockIdx.x / kXcd;
+    const int nwg = (gridDim.x - xcd + kXcd - 1) / kXcd;
+    const int64_t chunk = (ntask + kXcd - 1) / kXcd;
+    const int64_t t0 = xcd * chunk;
+    t1 = t0 + chunk < ntask ? t0 + chunk : ntask;
+    t = t0 + l;
+    tstep = nwg;
+  }
+  for (; t < t1; t += tstep) {
+    const TileTask tk = tasks[t];
+    const int col = tk.tile * W + w;
+    const int rr = wave * RPW + q;
+    const bool act = rr < tk.nrows && col < R;
+    const int64_t row = tk.row0 + rr;
+    int64_t e = 0, e1 = 0;
+    cd acc = {0.0, 0.0};
+    if (act) {
+      e = row_ptr[row];
+      e1 = row_ptr[row + 1];
+      const double2 xs = x[row * R + col];
+      const double dg = diag[row];
+      acc = {0.0 + dg * xs.x, 0.0 + dg * xs.y};
+    }
+    while (__ballot(e < e1)) {
+      int32_t cl[4];
+      uint32_t code[4];
+      double2 xv[4];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const bool ok = e + h < e1;
+        cl[h] = ok ? hcol[e + h] : 0;
+        code[h] = ok ? hcode[e + h] : 0u;
+      }
+#pragma unroll
+      for (int h = 0; h < 4; ++h) xv[h] = e + h < e1 ? x[(int64_t)cl[h] * R + col] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+        if (e + h < e1) acc = cadd(acc, cmul(hop_coef(code[h], ptab, spec), {xv[h].x, xv[h].y}));
+      e += 4;
+    }
+    if (act) y[row * R + col] = make_double2(acc.re, acc.im);
+  }
 }
 
 // Per-vector (R = 1) gather over a hop list: one row per thread.
@@ -824,12 +869,8 @@ __global__ void __launch_bounds__(256) k_apply_hl_r1(int64_t dim, const int64_t*
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[b]);
-    dot_out[0] = s;
-    *counter = 0u;
-  }
+  last_block_column_sums(partials, gridDim.x, 1, [&](int, double sum) { dot_out[0] = sum; });
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // Row-wise hop enumeration for CSR export and dense builds.
@@ -1113,6 +1154,34 @@ void launch_apply_hl(const HopList& h, int64_t dim, int R, int W, const int64_t*
   else
     k_apply_hl<4><<<(unsigned)grid, 256, 0, s>>>(dim, R, W, J, offs, n_canon, h.row_ptr, h.col, h.code, pt, sp,
                                                  h.diag, xx, yy, partials, counter, dot_out);
+}
+
+int tile_task_rows(int W) { return 4 * (64 / W); }
+
+void launch_apply_hl_tiles(const HopList& h, int R, int W, const TileTask* tasks, int64_t ntask, int cus,
+                           const double* x, double* y, hipStream_t s) {
+  if (ntask == 0) return;
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  double2* yy = reinterpret_cast<double2*>(y);
+  const double2* pt = reinterpret_cast<const double2*>(h.ptab);
+  const double2* sp = reinterpret_cast<const double2*>(h.spec);
+  int64_t grid = ntask;
+  if (grid > (int64_t)cus * 8) grid = (int64_t)cus * 8;
+  const unsigned g = (unsigned)grid;
+  switch (W) {
+    case 8:
+      k_apply_hl_tiles<8><<<g, 256, 0, s>>>(R, ntask, tasks, h.row_ptr, h.col, h.code, pt, sp, h.diag, xx, yy);
+      break;
+    case 16:
+      k_apply_hl_tiles<16><<<g, 256, 0, s>>>(R, ntask, tasks, h.row_ptr, h.col, h.code, pt, sp, h.diag, xx, yy);
+      break;
+    case 32:
+      k_apply_hl_tiles<32><<<g, 256, 0, s>>>(R, ntask, tasks, h.row_ptr, h.col, h.code, pt, sp, h.diag, xx, yy);
+      break;
+    default:
+      k_apply_hl_tiles<64><<<g, 256, 0, s>>>(R, ntask, tasks, h.row_ptr, h.col, h.code, pt, sp, h.diag, xx, yy);
+      break;
+  }
 }
 
 void launch_zero(double* p, int64_t n, hipStream_t s) {

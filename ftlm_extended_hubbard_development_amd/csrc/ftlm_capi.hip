@@ -268,6 +268,10 @@ struct ftlm_ctx {
   // hop list of the current block
   DevBuf hl_rowptr, hl_col, hl_code, hl_spec, hl_diag, hl_cnt, hl_scnt, hl_srowptr;
   int64_t hl_key = -1;
+  // column-tile tasks of the hop-list gather (TileTask), per block, R and W
+  DevBuf tile_tasks;
+  int64_t tile_key_gen = -1, n_tile_tasks = 0;
+  int tile_key_R = 0, tile_key_W = 0;
   int64_t block_gen = 0;
   HopList hl{};
   DevBuf item_tiles;
@@ -780,9 +784,43 @@ ApplyCfg resolve_cfg(ftlm_ctx* c, int R, int path, int w) {
 
 // y = H x for R columns with configuration cfg; partials/dot_out may be null.
 // Returns true when <x, y> was reduced into dot_out by the apply itself.
+// Tasks of the lane-dense tiled gather for W in {8, 16, 32, 64}: runs of
+// tile_task_rows(W) rows of one canonical-up group and one column tile,
+// ordered (group, tile, rows).
+const TileTask* ensure_tile_tasks(ftlm_ctx* c, int R, int W, int64_t* ntask) {
+  if (c->tile_key_gen != c->block_gen || c->tile_key_R != R || c->tile_key_W != W) {
+    std::vector<int64_t> offs(c->n_canon + 1);
+    HIPCHECK(hipMemcpy(offs.data(), c->cur.offs.p, 8 * (c->n_canon + 1), hipMemcpyDeviceToHost));
+    const int J = (R + W - 1) / W;
+    const int ch = tile_task_rows(W);
+    std::vector<TileTask> t;
+    for (int g = 0; g < c->n_canon; ++g)
+      for (int j = 0; j < J; ++j)
+        for (int64_t r0 = offs[g]; r0 < offs[g + 1]; r0 += ch)
+          t.push_back({r0, (int32_t)std::min<int64_t>(ch, offs[g + 1] - r0), j});
+    TileTask* d = c->tile_tasks.as<TileTask>(std::max<size_t>(t.size(), 1));
+    if (!t.empty()) HIPCHECK(hipMemcpy(d, t.data(), t.size() * sizeof(TileTask), hipMemcpyHostToDevice));
+    c->n_tile_tasks = (int64_t)t.size();
+    c->tile_key_gen = c->block_gen;
+    c->tile_key_R = R;
+    c->tile_key_W = W;
+  }
+  *ntask = c->n_tile_tasks;
+  return c->tile_tasks.ptr<TileTask>();
+}
+
+bool lane_dense_tile(int R, int w) { return w > 0 && w < R && (w == 8 || w == 16 || w == 32 || w == 64); }
+
 bool do_apply(ftlm_ctx* c, const ApplyCfg& cfg, int R, const double* x, double* y, double* partials,
               unsigned* counter, double* dot_out) {
   const int64_t dim = c->cur.dim;
+  if (cfg.path == 1 && R > 1 && lane_dense_tile(R, cfg.w)) {
+    const HopList& h = ensure_hops(c);
+    int64_t nt = 0;
+    const TileTask* tasks = ensure_tile_tasks(c, R, cfg.w, &nt);
+    launch_apply_hl_tiles(h, R, cfg.w, tasks, nt, c->cus, x, y, c->stream);
+    return false;
+  }
   if (cfg.path == 1) {
     const HopList& h = ensure_hops(c);
     const bool fused = cfg.w == 0 || R == 1;

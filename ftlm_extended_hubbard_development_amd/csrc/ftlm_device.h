@@ -266,6 +266,46 @@ __device__ __forceinline__ bool hop_target_ex(const BlockView& bv, const Lattice
   return true;
 }
 
+// Column sums of per-workgroup partials (partials[b * R + r], b < nb), run
+// by the last workgroup of a grid (256 threads) once every workgroup has
+// written its row. Columns are taken 256 at a time; the 256 / cols threads of
+// a column each add a strided share of the workgroups into eight independent
+// accumulators (so the loads overlap), and the shares are then added in
+// thread order: a fixed order, the same sums run to run. use(r, sum) is
+// called once per column, by one thread.
+template <typename F>
+__device__ __forceinline__ void last_block_column_sums(const double* __restrict__ partials, unsigned nb, int R,
+                                                       F&& use) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  for (int c0 = 0; c0 < R; c0 += 256) {
+    const int cols = R - c0 < 256 ? R - c0 : 256;
+    const int segs = 256 / cols;
+    const int q = t / cols, r = t % cols;
+    double acc = 0.0;
+    if (q < segs) {
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      unsigned b = (unsigned)q;
+      const unsigned step = (unsigned)segs;
+      for (; b + 7 * step < nb; b += 8 * step) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          a[u] += __builtin_nontemporal_load(&partials[(int64_t)(b + u * step) * R + c0 + r]);
+      }
+      for (; b < nb; b += step) a[0] += __builtin_nontemporal_load(&partials[(int64_t)b * R + c0 + r]);
+      acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    }
+    __syncthreads();  // the previous chunk's readers are done with sh
+    if (q < segs) sh[q * cols + r] = acc;
+    __syncthreads();
+    if (t < cols) {
+      double sum = 0.0;
+      for (int j = 0; j < segs; ++j) sum += sh[j * cols + t];
+      use(c0 + t, sum);
+    }
+  }
+}
+
 __device__ __forceinline__ bool hop_target(const BlockView& bv, const LatticeConst& g, int c_row, uint64_t up,
                                            uint64_t dn, double dnf_row, int spin, int from, int to, int dir,
                                            int64_t* col, cd* coef) {

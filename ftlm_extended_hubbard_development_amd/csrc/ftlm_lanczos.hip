@@ -22,6 +22,9 @@ namespace {
 
 inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// rows each thread of the one-column-per-thread recurrence keeps in flight
+constexpr int kRecurRows = 4;
+
 __global__ void k_lanczos_init(int R, LanczosState st) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
@@ -120,45 +123,49 @@ __global__ void k_normalize(int64_t n, int R, double2* __restrict__ q, const dou
 
 // Tree-order recurrence on unnormalised vectors. With q = s x, Hq = s y:
 //   x_next = s y - (alpha s) x - (bprev sprev) xp,  alpha = s^2 Re<x, y>
-// Column ownership is fixed per thread: for R <= 256 thread t owns column
-// t % R of rows t / R (+ multiples of 256/R); for larger R it owns columns
-// t, t+256, ... of every row. Per-column norm partials are summed per
-// workgroup in thread order and over workgroups in index order by the last
-// workgroup, which also applies the stopping rule (main.cpp:1043-1056) and
-// updates the scale factors.
+// Column ownership is fixed per thread: for R <= 256 (CPT = 1) thread t owns
+// column t % R of rows t / R (+ multiples of 256/R); for larger R (CPT = 4,
+// R <= 1024) it owns columns t, t+256, ... of every row. Each thread keeps U
+// rows in flight and adds its rows' |x_next|^2 in increasing row order.
+// Per-column norm partials are summed per workgroup in thread order and over
+// workgroups by the last workgroup, which also applies the stopping rule
+// (main.cpp:1043-1056) and updates the scale factors. The instantiation per
+// column count keeps the R <= 256 kernel at a few dozen registers (8 waves
+// per SIMD) instead of the four-column kernel's ~190.
+template <int CPT, int U>
 __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const double2* __restrict__ y,
                                                     const double2* __restrict__ x, double2* __restrict__ xp,
                                                     int k, int kdim, LanczosState st, double* partials,
                                                     unsigned* counter) {
-  __shared__ double sh[4][256];
+  __shared__ double sh[CPT][256];
   const int tid = threadIdx.x;
-  const int rpi = R <= 256 ? 256 / R : 1;         // rows per iteration
-  const int cpt = R <= 256 ? 1 : (R + 255) / 256;  // columns per thread
-  const int rsub = R <= 256 ? tid / R : 0;
-  const int r0 = R <= 256 ? tid % R : tid;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double s_[4], c2_[4], c3_[4];
-  bool run[4];
-  for (int j = 0; j < 4; ++j) {
+  const int rpi = CPT == 1 ? 256 / R : 1;   // rows per iteration
+  const int rsub = CPT == 1 ? tid / R : 0;
+  const int r0 = CPT == 1 ? tid % R : tid;
+  double acc[CPT], s_[CPT], c2_[CPT], c3_[CPT];
+  bool run[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
     const int r = r0 + 256 * j;
-    run[j] = rsub < rpi && j < cpt && r < R && !st.done[r];
+    acc[j] = 0.0;
+    s_[j] = c2_[j] = c3_[j] = 0.0;
+    run[j] = rsub < rpi && r < R && !st.done[r];
     if (run[j]) {
-      const double s = st.scale[r];
-      s_[j] = s;
-      c2_[j] = (s * s * st.dot[r]) * s;
+      const double sc = st.scale[r];
+      s_[j] = sc;
+      c2_[j] = (sc * sc * st.dot[r]) * sc;
       c3_[j] = st.bprev[r] * st.sprev[r];
     }
   }
-  // four rows in flight per thread; each row's elements are processed in the
-  // same order as a one-row loop would, so the sums do not change
   const int64_t stride = (int64_t)gridDim.x * rpi;
-  for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += 4 * stride) {
-    for (int j = 0; j < 4; ++j) {
-      if (!run[j]) continue;
-      double2 vy[4], vx[4], vp[4];
-      int64_t e[4];
+  for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += U * stride) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+    for (int j = 0; j < CPT; ++j) {
+      if (!run[j]) continue;
+      double2 vy[U], vx[U], vp[U];
+      int64_t e[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         e[u] = (row + u * stride) * R + r0 + 256 * j;
         if (row + u * stride < dim) {
           vy[u] = y[e[u]];
@@ -167,7 +174,7 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         if (row + u * stride >= dim) continue;
         const double re = (s_[j] * vy[u].x - c2_[j] * vx[u].x) - c3_[j] * vp[u].x;
         const double im = (s_[j] * vy[u].y - c2_[j] * vx[u].y) - c3_[j] * vp[u].y;
@@ -176,16 +183,17 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
       }
     }
   }
-  for (int j = 0; j < 4; ++j) sh[j][tid] = acc[j];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) sh[j][tid] = acc[j];
   __syncthreads();
   for (int r = tid; r < R; r += 256) {
-    double s = 0.0;
-    if (R <= 256) {
-      for (int q = r; q < rpi * R; q += R) s += sh[0][q];
+    double sum = 0.0;
+    if (CPT == 1) {
+      for (int q = r; q < rpi * R; q += R) sum += sh[0][q];
     } else {
-      s = sh[r / 256][r % 256];
+      sum = sh[r / 256][r % 256];
     }
-    partials[(int64_t)blockIdx.x * R + r] = s;
+    partials[(int64_t)blockIdx.x * R + r] = sum;
   }
   __threadfence();
   __syncthreads();
@@ -194,10 +202,8 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int r = tid; r < R; r += 256) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
-    if (st.done[r]) continue;
+  last_block_column_sums(partials, gridDim.x, R, [&](int r, double s) {
+    if (st.done[r]) return;
     const double sc = st.scale[r];
     st.alpha[(int64_t)k * R + r] = sc * sc * st.dot[r];
     st.mcount[r] = k + 1;
@@ -205,32 +211,37 @@ __global__ void __launch_bounds__(256) k_recur_tree(int64_t dim, int R, const do
     if (b < 1e-12 || k + 1 == kdim) {
       st.done[r] = 1;
       st.scale[r] = 0.0;
-      continue;
+      return;
     }
     st.beta[(int64_t)k * R + r] = b;
     st.sprev[r] = sc;
     st.scale[r] = 1.0 / b;
     st.bprev[r] = b;
-  }
+  });
   if (tid == 0) *counter = 0u;
 }
 
 // Per-column Re<x, y> with the column ownership of k_recur_tree.
+template <int CPT>
 __global__ void __launch_bounds__(256) k_dot_tree(int64_t dim, int R, const double2* __restrict__ x,
                                                   const double2* __restrict__ y, double* partials, unsigned* counter,
                                                   double* out) {
-  __shared__ double sh[4][256];
+  __shared__ double sh[CPT][256];
   const int tid = threadIdx.x;
-  const int rpi = R <= 256 ? 256 / R : 1;
-  const int cpt = R <= 256 ? 1 : (R + 255) / 256;
-  const int rsub = R <= 256 ? tid / R : 0;
-  const int r0 = R <= 256 ? tid % R : tid;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  bool run[4];
-  for (int j = 0; j < 4; ++j) run[j] = rsub < rpi && j < cpt && r0 + 256 * j < R;
+  const int rpi = CPT == 1 ? 256 / R : 1;
+  const int rsub = CPT == 1 ? tid / R : 0;
+  const int r0 = CPT == 1 ? tid % R : tid;
+  double acc[CPT];
+  bool run[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    acc[j] = 0.0;
+    run[j] = rsub < rpi && r0 + 256 * j < R;
+  }
   const int64_t stride = (int64_t)gridDim.x * rpi;
   for (int64_t row = (int64_t)blockIdx.x * rpi + rsub; row < dim; row += 4 * stride) {
-    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
       if (!run[j]) continue;
       double2 a[4], b[4];
 #pragma unroll
@@ -246,16 +257,17 @@ __global__ void __launch_bounds__(256) k_dot_tree(int64_t dim, int R, const doub
         if (row + u * stride < dim) acc[j] += a[u].x * b[u].x + a[u].y * b[u].y;
     }
   }
-  for (int j = 0; j < 4; ++j) sh[j][tid] = acc[j];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) sh[j][tid] = acc[j];
   __syncthreads();
   for (int r = tid; r < R; r += 256) {
-    double s = 0.0;
-    if (R <= 256) {
-      for (int q = r; q < rpi * R; q += R) s += sh[0][q];
+    double sum = 0.0;
+    if (CPT == 1) {
+      for (int q = r; q < rpi * R; q += R) sum += sh[0][q];
     } else {
-      s = sh[r / 256][r % 256];
+      sum = sh[r / 256][r % 256];
     }
-    partials[(int64_t)blockIdx.x * R + r] = s;
+    partials[(int64_t)blockIdx.x * R + r] = sum;
   }
   __threadfence();
   __syncthreads();
@@ -264,11 +276,7 @@ __global__ void __launch_bounds__(256) k_dot_tree(int64_t dim, int R, const doub
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int r = tid; r < R; r += 256) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += __builtin_nontemporal_load(&partials[(int64_t)b * R + r]);
-    out[r] = s;
-  }
+  last_block_column_sums(partials, gridDim.x, R, [&](int r, double s) { out[r] = s; });
   if (tid == 0) *counter = 0u;
 }
 
@@ -690,15 +698,23 @@ void launch_normalize(int64_t dim, int R, double* q, const double* beta_k, const
 
 void launch_recur_tree(int64_t dim, int R, const double* y, const double* x, double* xp, int k, int kdim,
                        LanczosState st, double* partials, unsigned* counter, int grid, hipStream_t s) {
-  k_recur_tree<<<grid, 256, 0, s>>>(
-      dim, R, reinterpret_cast<const double2*>(y), reinterpret_cast<const double2*>(x),
-      reinterpret_cast<double2*>(xp), k, kdim, st, partials, counter);
+  const double2* yy = reinterpret_cast<const double2*>(y);
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  double2* pp = reinterpret_cast<double2*>(xp);
+  if (R <= 256)
+    k_recur_tree<1, kRecurRows><<<grid, 256, 0, s>>>(dim, R, yy, xx, pp, k, kdim, st, partials, counter);
+  else
+    k_recur_tree<4, 4><<<grid, 256, 0, s>>>(dim, R, yy, xx, pp, k, kdim, st, partials, counter);
 }
 
 void launch_dot_tree(int64_t dim, int R, const double* x, const double* y, double* partials, unsigned* counter,
                      double* out, int grid, hipStream_t s) {
-  k_dot_tree<<<grid, 256, 0, s>>>(dim, R, reinterpret_cast<const double2*>(x), reinterpret_cast<const double2*>(y),
-                                   partials, counter, out);
+  if (R <= 256)
+    k_dot_tree<1><<<grid, 256, 0, s>>>(dim, R, reinterpret_cast<const double2*>(x),
+                                       reinterpret_cast<const double2*>(y), partials, counter, out);
+  else
+    k_dot_tree<4><<<grid, 256, 0, s>>>(dim, R, reinterpret_cast<const double2*>(x),
+                                       reinterpret_cast<const double2*>(y), partials, counter, out);
 }
 
 void launch_transpose_in(int64_t dim, int R, int r0, int cnt, const double* src, double* x, hipStream_t s) {

@@ -80,6 +80,20 @@ void launch_hops_fill(const LatticeConst* g, BlockView bv, const int64_t* row_pt
                       int32_t* hcol, uint32_t* hcode, double* spec, double* diag, hipStream_t s);
 void launch_apply_hl(const HopList& h, int64_t dim, int R, int W, const int64_t* offs, int n_canon, int cus,
                      const double* x, double* y, double* partials, unsigned* counter, double* dot_out, hipStream_t s);
+// Column-tiled hop-list gather for tiles of W <= 64 columns: 64 / W rows per
+// wavefront, lane = (row in the wavefront, column in the tile). A task is a
+// run of 4 * 64 / W consecutive rows of one canonical-up group and one
+// column tile; tasks are ordered (group, tile, rows) and cut into one
+// contiguous range per XCD, so the workgroups of an XCD gather the down-spin
+// hops of one group's tile slab (rows * W * 16 B) from its L2.
+struct TileTask {
+  int64_t row0;
+  int32_t nrows;
+  int32_t tile;
+};
+int tile_task_rows(int W);
+void launch_apply_hl_tiles(const HopList& h, int R, int W, const TileTask* tasks, int64_t ntask, int cus,
+                           const double* x, double* y, hipStream_t s);
 // CSR export: counts per row (pass 1) and entries (pass 2).
 void launch_csr_count(const LatticeConst* g, BlockView bv, int all_dirs, int32_t* counts, hipStream_t s);
 void launch_csr_fill(const LatticeConst* g, BlockView bv, int all_dirs, const int64_t* row_ptr, int32_t* col,

@@ -74,3 +74,20 @@ def test_gain_against_the_exact_answer(gain):
     # faster at a worst-case error within twice FTLM's own
     assert loose["speedup"] >= 3.0
     assert loose["dn_exact_max"] <= 2.0 * gain["ftlm_dn_exact_max"]
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "measured on this 8 x 8 scan: no tolerance of the judged scan reaches a worst-case error vs the exact answer at "
+    "or below that of FTLM with R = 16 on every twist (the cheapest FTLM variant here, 1.07x faster than R = 32) "
+    "while being faster than it; the one 16 x 16 scan where it did (tolerance 2e-2: 3.5e-2 at 2.77x vs 3.5e-2 at "
+    "1.07x) did not reproduce here"))
+def test_gain_at_equal_worst_case_error(gain):
+    """The claim a user needs: some tolerance makes the judged scan at least
+    as accurate in the worst case as the cheapest FTLM scan without the
+    network, and faster than it."""
+    cheap = min(gain["ftlm_fewer_samples"], key=lambda c: c["t"])
+    wins = [c for c in gain["tolerance_curve"]
+            if c["dn_exact_max"] <= cheap["dn_exact_max"] and c["speedup"] > cheap["speedup"]]
+    say(f"cheapest FTLM: R={cheap['samples']} worst {cheap['dn_exact_max']:.3e} at {cheap['speedup']:.2f}x; "
+        f"tolerances winning at equal worst-case error: {[(c['tol'], c['speedup']) for c in wins]}")
+    assert wins
