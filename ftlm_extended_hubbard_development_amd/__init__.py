@@ -80,6 +80,8 @@ SIGNATURES = {
     "ftlm_pack_tune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
     "ftlm_pack_info": (C.c_int, [C.c_void_p, _lp, _up]),
     "ftlm_thermo": (C.c_int, [C.c_void_p, C.c_int64, _lp, _dp, _dp, _ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "ftlm_thermo_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ftlm_set_start_source": (C.c_int, [C.c_void_p, C.c_int]),
     "ftlm_exact_queue_begin": (C.c_int, [C.c_void_p, C.c_uint64]),
     "ftlm_exact_queue_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _lp]),

@@ -1338,6 +1338,9 @@ int ftlm_create(const ftlm_model* model, int device, ftlm_ctx** out) {
     HIPCHECK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    // device address of the fixed scratch array of ftlm_thermo_noalloc on
+    // this device (resolved once per device, with the device current)
+    HIPCHECK((hipError_t)thermo_scratch_setup(device));
     c->model = *model;
     build_lattice(c.get());
     upload_ctx_tables(c.get());
@@ -1967,6 +1970,28 @@ int ftlm_thermo(ftlm_ctx* ctx, int64_t nblk, const int64_t* off, const double* e
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(out, dout, (size_t)np * 4 * 8, hipMemcpyDeviceToHost, s));
     sync(ctx);
+  });
+}
+
+int ftlm_thermo_noalloc(ftlm_ctx* ctx, int64_t nv, const double* d_eig, const double* d_weight,
+                        const uint8_t* d_particles, int nbeta, const double* d_betas, int nmu, const double* d_mu,
+                        double* d_out, void* stream) {
+  return guarded([&] {
+    if (!ctx || !d_betas || !d_mu || !d_out) throw Error(FTLM_ERR_ARG, "null argument");
+    if (nv < 0 || (nv > 0 && (!d_eig || !d_weight || !d_particles))) throw Error(FTLM_ERR_ARG, "null spectrum");
+    if (nbeta <= 0 || nmu <= 0) throw Error(FTLM_ERR_ARG, "empty beta or mu list");
+    if ((int64_t)nbeta * nmu > (1 << 24) ||
+        thermo_noalloc_doubles(nv, nbeta, nmu) > kThermoScratchDoubles)
+      throw Error(FTLM_ERR_LIMIT,
+                  "ftlm_thermo_noalloc: 3 * ceil(nv / 4096) * nbeta * nmu partial sums exceed its fixed 32 MiB "
+                  "scratch (use ftlm_thermo, or fewer values per call)");
+    double* scr = thermo_scratch(ctx->device);
+    if (!scr) throw Error(FTLM_ERR_STATE, "scratch of the device not set up");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    launch_thermo_noalloc(d_eig, d_weight, d_particles, nv, d_betas, nbeta, d_mu, nmu, scr, d_out, s);
+    HIPCHECK(hipGetLastError());
+    if (!stream) HIPCHECK(hipStreamSynchronize(s));
   });
 }
 

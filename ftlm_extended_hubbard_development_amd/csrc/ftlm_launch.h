@@ -185,6 +185,18 @@ void launch_start_vectors(uint64_t bseed, int r0, int col0, int cnt, int R, int6
 // part: thermo_workgroups(nv) * nb * nm * 3 doubles; out: nb * nm * 4
 // (emin, z, n1, n2), pair p = beta index * nm + mu index.
 int64_t thermo_workgroups(int64_t nv);
+// allocation-free variant: a fixed device array (kThermoScratchDoubles, one
+// copy per GPU) holds the partials and the per-mu minima, which are computed
+// on the device. thermo_scratch_setup resolves the array's device address
+// with the device current (once per device); thermo_scratch returns it
+// (nullptr before setup).
+constexpr int64_t kThermoScratchDoubles = (int64_t)4 << 20;  // 32 MiB
+constexpr int kMaxDevices = 64;
+int thermo_scratch_setup(int device);
+double* thermo_scratch(int device);
+int64_t thermo_noalloc_doubles(int64_t nv, int nb, int nm);
+void launch_thermo_noalloc(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas,
+                           int nb, const double* mus, int nm, double* scr, double* out, hipStream_t s);
 void launch_thermo(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas, int nb,
                    const double* mus, int nm, const double* emin, double* part, double* out, hipStream_t s);
 

@@ -21,9 +21,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "ftlm_launch.h"
 
 namespace ftlm {
+
+// Device scratch of the allocation-free entry point (ftlm_thermo_noalloc):
+// workgroup partials and the per-mu minima live in this fixed array, one
+// copy per GPU, held until the process ends. Host code uses the address
+// hipGetSymbolAddress returns for the device (thermo_scratch_setup), never
+// the array's name.
+static __device__ __align__(256) double scratch[kThermoScratchDoubles];
+
 namespace {
 
 constexpr int kV = 16;              // values per lane
@@ -94,6 +105,43 @@ __global__ __launch_bounds__(256) void k_thermo(const double* __restrict__ E, co
   }
 }
 
+// part_min[g * nm + k] = min over the workgroup's values of E - mu_k N (the
+// same tiling as k_thermo). E - c is monotone in E, so the minimum equals
+// min_blocks (min_j E_j) - mu N_b computed on the host by ftlm_thermo.
+__global__ __launch_bounds__(256) void k_emin_part(const double* __restrict__ E, const uint8_t* __restrict__ N,
+                                                   int64_t nv, const double* __restrict__ mus, int nm,
+                                                   double* __restrict__ part_min) {
+  __shared__ double red[256];
+  const int64_t v0 = (int64_t)blockIdx.x * kWg + threadIdx.x;
+  double e[kV], n[kV];
+  for (int j = 0; j < kV; ++j) {
+    const int64_t i = v0 + (int64_t)j * 256;
+    e[j] = i < nv ? E[i] : __builtin_inf();
+    n[j] = i < nv ? (double)N[i] : 0.0;
+  }
+  for (int k = 0; k < nm; ++k) {
+    const double mu = mus[k];
+    double m = __builtin_inf();
+    for (int j = 0; j < kV; ++j) m = fmin(m, e[j] - mu * n[j]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part_min[(int64_t)blockIdx.x * nm + k] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_emin_final(const double* __restrict__ part_min, int nwg, int nm, double* __restrict__ emin) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nm) return;
+  double m = __builtin_inf();
+  for (int g = 0; g < nwg; ++g) m = fmin(m, part_min[(int64_t)g * nm + k]);
+  emin[k] = m;
+}
+
 // out[p] = (emin, z, n1, n2) summing the workgroup partials in index order.
 __global__ void k_thermo_reduce(const double* __restrict__ part, int nwg, int np, int nm,
                                 const double* __restrict__ emin, double* __restrict__ out) {
@@ -110,6 +158,49 @@ __global__ void k_thermo_reduce(const double* __restrict__ part, int nwg, int np
 }  // namespace
 
 int64_t thermo_workgroups(int64_t nv) { return (nv + kWg - 1) / kWg; }
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+int thermo_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return (int)hipSuccess;
+  double* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return (int)e;
+  g_scratch_dev[device] = p;
+  return (int)hipSuccess;
+}
+
+double* thermo_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
+int64_t thermo_noalloc_doubles(int64_t nv, int nb, int nm) {
+  const int64_t nwg = thermo_workgroups(nv);
+  return std::max<int64_t>(nwg, 1) * nb * nm * 3 + std::max<int64_t>(nwg, 1) * nm + nm;
+}
+
+void launch_thermo_noalloc(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas,
+                           int nb, const double* mus, int nm, double* scr, double* out, hipStream_t s) {
+  const int64_t nwg = thermo_workgroups(nv);
+  double* part = scr;
+  double* part_min = part + std::max<int64_t>(nwg, 1) * nb * nm * 3;
+  double* emin = part_min + std::max<int64_t>(nwg, 1) * nm;
+  if (nwg > 0) {
+    hipLaunchKernelGGL(k_emin_part, dim3((unsigned)nwg), dim3(256), 0, s, E, N, nv, mus, nm, part_min);
+    hipLaunchKernelGGL(k_emin_final, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, part_min, (int)nwg, nm,
+                       emin);
+  } else {
+    hipLaunchKernelGGL(k_emin_final, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, part_min, 0, nm, emin);
+  }
+  launch_thermo(E, W, N, nv, betas, nb, mus, nm, emin, part, out, s);
+}
 
 void launch_thermo(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas, int nb,
                    const double* mus, int nm, const double* emin, double* part, double* out, hipStream_t s) {

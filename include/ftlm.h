@@ -234,6 +234,26 @@ int ftlm_thermo(ftlm_ctx* ctx, int64_t nblk, const int64_t* off, const double* e
                 const int32_t* particles, const double* prefactor, int nbeta, const double* betas, int nmu,
                 const double* mu, double* out);
 
+/* ftlm_thermo without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The spectra are already in device memory, one entry per value: d_eig
+ * (Ritz value or eigenvalue), d_weight (its block's prefactor times its
+ * weight: dim * w for FTLM blocks, 1 for exact ones) and d_particles (its
+ * block's particle number); d_betas, d_mu and d_out (nbeta * nmu * 4
+ * doubles, laid out as in ftlm_thermo) are device pointers too. The running
+ * minimum of E - mu N is found on the device, so d_out equals ftlm_thermo's
+ * output for the same spectra bit for bit. Partial sums live in a fixed
+ * 32 MiB device array (one per GPU, kept until the process ends): calls with
+ * 3 * ceil(nv / 4096) * nbeta * nmu + (ceil(nv / 4096) + 1) * nmu doubles
+ * above 4 Mi return FTLM_ERR_LIMIT. Calls on one GPU share that array, so
+ * calls that overlap on one GPU (other host threads, other streams) race.
+ * stream: NULL runs on the context's stream and returns when done; a
+ * hipStream_t enqueues the work there and returns at once. */
+int ftlm_thermo_noalloc(ftlm_ctx* ctx, int64_t nv, const double* d_eig, const double* d_weight,
+                        const uint8_t* d_particles, int nbeta, const double* d_betas, int nmu, const double* d_mu,
+                        double* d_out, void* stream);
+
 /* diagonalize_block_exact for blocks (mx[i], my[i]) of the current sector,
  * batched in one launch: eig_out receives the ascending eigenvalues of each
  * block back to back, offsets[i]..offsets[i+1]. */
