@@ -250,6 +250,20 @@ def sweep_estimate(ftlm, R=128, m=80, pack_every=8, validate=0, log=None):
             "start_vectors_s_sampled": round(t_host_starts, 2), "checks": checks}
 
 
+def memory_ceiling(eng, R, rows=10353252, k=35, reps=3):
+    """HBM rates of the apply's access shapes without its hop metadata
+    (ftlm_memprobe): stream read and copy of rows x R complex doubles, and a
+    gather writing each row as the sum of k hashed rows (k = 35 ~ the 34.13
+    hop targets + the row itself of the (8,8) block, one wavefront per row
+    like k_apply_hl)."""
+    out = {"rows": rows, "width": R, "row_bytes": R * 16}
+    for kind, name in ((0, "read"), (1, "copy"), (2, "gather")):
+        ms, by = eng.memprobe(kind, rows, R, k, reps)
+        out[name] = {"ms": round(ms, 3), "GB": round(by / 1e9, 2), "TBps": round(by / ms / 1e9, 3)}
+        print(f"# ceiling {name}: {ms:.2f} ms, {by / ms / 1e9:.2f} TB/s", file=sys.stderr, flush=True)
+    return out
+
+
 def read_traffic():
     """HBM bytes per H-apply launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -277,6 +291,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES bytes (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters to HBM bytes)")
+    ap.add_argument("--ceiling", action="store_true",
+                    help="only measure what the HBM path delivers for the apply's shapes (stream read, copy, random "
+                         "2 KB-row gather of 35 rows per row) on the workload block's vector size, and print it")
     ap.add_argument("--tile", default="auto",
                     help="H-apply configuration: auto (autotuner) or PATH,W (path 1 hop list / 2 on the fly; "
                          "W -1 automatic, 0 untiled, >0 tile width)")
@@ -303,6 +320,10 @@ def main():
 
     nup, ndn, mx, my = (int(t) for t in args.block.split(","))
     eng = ftlm.Engine(lx=4, ly=4, u=7.0)
+    if args.ceiling:
+        print(json.dumps(memory_ceiling(eng, args.R)), flush=True)
+        eng.close()
+        return
     t0 = time.perf_counter()
     eng.sector(nup, ndn)
     dim = eng.block(mx, my)

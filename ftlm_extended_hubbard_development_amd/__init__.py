@@ -82,6 +82,7 @@ SIGNATURES = {
     "ftlm_thermo": (C.c_int, [C.c_void_p, C.c_int64, _lp, _dp, _dp, _ip, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
     "ftlm_thermo_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                       C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ftlm_memprobe": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, _dp, _dp]),
     "ftlm_set_start_source": (C.c_int, [C.c_void_p, C.c_int]),
     "ftlm_exact_queue_begin": (C.c_int, [C.c_void_p, C.c_uint64]),
     "ftlm_exact_queue_add": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _lp]),
@@ -361,6 +362,14 @@ class Engine:
         b = C.c_uint64(0)
         self._check(self.lib.ftlm_pack_info(self.h, C.byref(n), C.byref(b)))
         return n.value, b.value
+
+    def memprobe(self, kind, rows, width, k=36, reps=3):
+        """HBM probe with the batched apply's shapes (kind 0 read, 1 copy, 2
+        random gather of k rows per output row): (ms per launch, bytes)."""
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        self._check(self.lib.ftlm_memprobe(self.h, kind, rows, width, k, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def set_start_source(self, source):
         """START_HOST (libstdc++ stream on host threads, bit-identical to the

@@ -1995,6 +1995,37 @@ int ftlm_thermo_noalloc(ftlm_ctx* ctx, int64_t nv, const double* d_eig, const do
   });
 }
 
+int ftlm_memprobe(ftlm_ctx* ctx, int kind, int64_t rows, int width, int k, int reps, double* ms_per_launch,
+                  double* bytes_per_launch) {
+  return guarded([&] {
+    if (!ctx || !ms_per_launch || !bytes_per_launch) throw Error(FTLM_ERR_ARG, "null argument");
+    if (kind < 0 || kind > 2) throw Error(FTLM_ERR_ARG, "kind must be 0 (read), 1 (copy) or 2 (gather)");
+    if (rows < 1 || width < 1 || width > 256 || reps < 1 || (kind == 2 && k < 1))
+      throw Error(FTLM_ERR_ARG, "rows, reps and k must be positive, width in [1, 256]");
+    HIPCHECK(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)rows * width * 16;
+    DevBuf x, y;
+    double* dx = x.as<double>(2 * (size_t)rows * width);
+    double* dy = y.as<double>(2 * (size_t)rows * width);
+    launch_zero(dx, 2 * rows * (int64_t)width, ctx->stream);
+    launch_memprobe(kind, rows, width, k, ctx->cus, dx, dy, ctx->stream);  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < reps; ++i) launch_memprobe(kind, rows, width, k, ctx->cus, dx, dy, ctx->stream);
+    HIPCHECK(hipEventRecord(e1, ctx->stream));
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = ms / reps;
+    *bytes_per_launch = kind == 0 ? (double)bytes : kind == 1 ? 2.0 * bytes : (double)bytes * (k + 1);
+  });
+}
+
 int ftlm_pack_info(ftlm_ctx* ctx, int64_t* entries, uint64_t* bytes) {
   return guarded([&] {
     if (!ctx) throw Error(FTLM_ERR_ARG, "null context");

@@ -200,6 +200,11 @@ void launch_thermo_noalloc(const double* E, const double* W, const uint8_t* N, i
 void launch_thermo(const double* E, const double* W, const uint8_t* N, int64_t nv, const double* betas, int nb,
                    const double* mus, int nm, const double* emin, double* part, double* out, hipStream_t s);
 
+// ---- memory-system probes (ftlm_probe.hip) ----
+// kind 0 stream read, 1 stream copy, 2 random-row gather of k rows per
+// output row; rows x width complex doubles in x (and y).
+void launch_memprobe(int kind, int64_t rows, int width, int k, int cus, const double* x, double* y, hipStream_t s);
+
 // ---- exact (ftlm_exact.hip) ----
 void launch_herm_eig(const int64_t* moff, const int64_t* dims, int nblk, double* mats, double* work,
                      double* eig_out, const int64_t* eoff, hipStream_t s);

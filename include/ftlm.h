@@ -260,6 +260,18 @@ int ftlm_thermo_noalloc(ftlm_ctx* ctx, int64_t nv, const double* d_eig, const do
 int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t* my, double* eig_out,
                       int64_t* offsets);
 
+/* Memory-system probe for the benchmark (no reference counterpart): what the
+ * GPU's HBM path delivers for the batched apply's access shapes without its
+ * hop metadata and arithmetic, on `rows` rows of `width` (<= 256) complex
+ * doubles. kind 0 streams the rows in (read only), 1 copies them, 2 writes
+ * every row as the sum of `k` source rows chosen by a fixed hash (one
+ * wavefront per row, like the batched apply). Allocates two buffers of
+ * rows * width * 16 bytes for the call, times `reps` launches after one
+ * warm-up launch; *ms_per_launch and *bytes_per_launch (read + written
+ * bytes: rows * width * 16 times 1, 2 or k + 1) are returned. */
+int ftlm_memprobe(ftlm_ctx* ctx, int kind, int64_t rows, int width, int k, int reps, double* ms_per_launch,
+                  double* bytes_per_launch);
+
 /* ---- Learned surrogate (no reference counterpart) -------------------------
  * An MLP maps (U, V, t', phix, phiy, beta, mu) to (n, compressibility), the
  * two observables the reference's CSV reports per mu (main.cpp:1409-1455),
