@@ -92,6 +92,60 @@ __global__ void __launch_bounds__(1024) k_prefix64(const int64_t* __restrict__ i
   if (tid == 1023) out[n] = part[1023];
 }
 
+// Multi-workgroup exclusive scan for long arrays: tiles of 4096 elements.
+// Pass 1 sums each tile (coalesced), the tile sums are scanned by
+// k_prefix64, pass 3 scans each tile locally (16 consecutive elements per
+// thread, then the thread totals in LDS) and adds the tile's offset.
+constexpr int kScanTile = 4096;
+
+__global__ void __launch_bounds__(256) k_tile_sums(const int64_t* __restrict__ in, int64_t n,
+                                                   int64_t* __restrict__ sums) {
+  __shared__ int64_t red[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int64_t s = 0;
+  for (int j = 0; j < kScanTile / 256; ++j) {
+    const int64_t i = base + j * 256 + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_tile_scan(const int64_t* __restrict__ in, int64_t n,
+                                                   const int64_t* __restrict__ tile_off, int64_t* __restrict__ out) {
+  __shared__ int64_t part[256];
+  constexpr int kPer = kScanTile / 256;
+  const int tid = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * kScanTile + (int64_t)tid * kPer;
+  int64_t v[kPer];
+  int64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    v[j] = b0 + j < n ? in[b0 + j] : 0;
+    s += v[j];
+  }
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int64_t a = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += a;
+    __syncthreads();
+  }
+  int64_t run = tile_off[blockIdx.x] + part[tid] - s;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    if (b0 + j < n) out[b0 + j] = run;
+    run += v[j];
+  }
+  if (b0 < n && n <= b0 + kPer) out[n] = run;  // the thread holding the last element
+}
+
 __global__ void k_canon_fill(int64_t n_upc, int nup, int L, const uint64_t* __restrict__ binom,
                              const uint8_t* __restrict__ is_canon, const int32_t* __restrict__ cidx,
                              const int64_t* __restrict__ min_rank, const uint64_t* __restrict__ amask,
@@ -233,6 +287,20 @@ void launch_segscan(const uint8_t* flags, int64_t seg_len, int nseg, int32_t* ou
 
 void launch_prefix64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) {
   k_prefix64<<<1, 1024, 0, s>>>(in, n, out);
+}
+
+int64_t prefix64_tmp_words(int64_t n) { return 2 * ((n + kScanTile - 1) / kScanTile) + 1; }
+
+void launch_prefix64_tiled(const int64_t* in, int64_t n, int64_t* out, int64_t* tmp, hipStream_t s) {
+  if (n <= kScanTile) {
+    k_prefix64<<<1, 1024, 0, s>>>(in, n, out);
+    return;
+  }
+  const int64_t nt = (n + kScanTile - 1) / kScanTile;
+  // tmp[0..nt): tile sums, then their exclusive scan in tmp[nt..2nt] (+ total)
+  k_tile_sums<<<(unsigned)nt, 256, 0, s>>>(in, n, tmp);
+  k_prefix64<<<1, 1024, 0, s>>>(tmp, nt, tmp + nt);
+  k_tile_scan<<<(unsigned)nt, 256, 0, s>>>(in, n, tmp + nt, out);
 }
 
 void launch_canon_fill(int64_t n_upc, int nup, int L, const uint64_t* binom, const uint8_t* is_canon,

@@ -266,7 +266,7 @@ struct ftlm_ctx {
   // R = 1 (per-vector) runs: the on-the-fly kernel is the faster plain path
   int apply_path_r1 = 2;
   // hop list of the current block
-  DevBuf hl_rowptr, hl_col, hl_code, hl_spec, hl_diag, hl_cnt, hl_scnt, hl_srowptr;
+  DevBuf hl_rowptr, hl_col, hl_code, hl_spec, hl_diag, hl_cnt, hl_scnt, hl_srowptr, hl_scan_tmp;
   int64_t hl_key = -1;
   // column-tile tasks of the hop-list gather (TileTask), per block, R and W
   DevBuf tile_tasks;
@@ -739,8 +739,9 @@ const HopList& ensure_hops(ftlm_ctx* c) {
   int64_t* rp = c->hl_rowptr.as<int64_t>(n + 1);
   int64_t* srp = c->hl_srowptr.as<int64_t>(n + 1);
   launch_hops_count(gdev(c), b.view, cnt, scnt, s);
-  launch_prefix64(cnt, n, rp, s);
-  launch_prefix64(scnt, n, srp, s);
+  int64_t* tmp = c->hl_scan_tmp.as<int64_t>((size_t)prefix64_tmp_words(n));
+  launch_prefix64_tiled(cnt, n, rp, tmp, s);
+  launch_prefix64_tiled(scnt, n, srp, tmp, s);
   HIPCHECK(hipGetLastError());
   const int64_t nnz = read1<int64_t>(c, rp + n);
   const int64_t nsp = read1<int64_t>(c, srp + n);
@@ -1231,8 +1232,9 @@ bool pack_add(ftlm_ctx* c, uint64_t bseed, int64_t* index) {
   int64_t* srp = c->hl_srowptr.as<int64_t>(n + 1);
   c->hl_key = -1;  // the ctx hop-list buffers are reused here
   launch_hops_count(gdev(c), b.view, cnt, scnt, s);
-  launch_prefix64(cnt, n, rp_tmp, s);
-  launch_prefix64(scnt, n, srp, s);
+  int64_t* tmp = c->hl_scan_tmp.as<int64_t>((size_t)prefix64_tmp_words(n));
+  launch_prefix64_tiled(cnt, n, rp_tmp, tmp, s);
+  launch_prefix64_tiled(scnt, n, srp, tmp, s);
   HIPCHECK(hipGetLastError());
   int64_t tot[2];
   HIPCHECK(hipMemcpyAsync(&tot[0], rp_tmp + n, 8, hipMemcpyDeviceToHost, s));

@@ -22,6 +22,10 @@ void launch_segscan(const uint8_t* flags, int64_t seg_len, int nseg, int32_t* ou
                     hipStream_t s);
 // Exclusive prefix sum of n int64 values into out[0..n] (out[n] = total).
 void launch_prefix64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
+// The same exclusive scan (out[n] = total) over many workgroups for long
+// arrays; tmp holds prefix64_tmp_words(n) int64.
+int64_t prefix64_tmp_words(int64_t n);
+void launch_prefix64_tiled(const int64_t* in, int64_t n, int64_t* out, int64_t* tmp, hipStream_t s);
 void launch_canon_fill(int64_t n_upc, int nup, int L, const uint64_t* binom, const uint8_t* is_canon,
                        const int32_t* cidx_of_rank, const int64_t* min_rank, const uint64_t* amask,
                        int32_t* upc_canon, uint64_t* can_bits, uint64_t* can_stab, uint8_t* nt_flag,
