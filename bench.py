@@ -264,6 +264,56 @@ def memory_ceiling(eng, R, rows=10353252, k=35, reps=3):
     return out
 
 
+# FLOPs of one surrogate training step at batch B, as the kernels execute
+# it (features padded to 32, outputs to 32): forward 2B(32*256 + 3*256^2 +
+# 256*32), backward the data gradients of layers 4..1 and the weight
+# gradients of layers 4..0 (the same products again, minus the first layer's
+# data gradient).
+def surrogate_step_flop(B):
+    fwd = 2 * B * (32 * 256 + 3 * 256 * 256 + 256 * 32)
+    dgrad = 2 * B * (256 * 32 + 3 * 256 * 256)
+    wgrad = 2 * B * (32 * 256 + 3 * 256 * 256 + 256 * 32)
+    return fwd + dgrad + wgrad
+
+
+def surrogate_step(steps, eager, B=4096, n=200000):
+    """Training steps of the surrogate MLP (bf16 MFMA kernels, fp32
+    accumulation) at batch B on synthetic rows over its parameter box:
+    replayed HIP graphs (what training runs), or eager launches (`eager`:
+    for kernel-trace profiling, which does not follow graph replays)."""
+    import numpy as np
+
+    from ftlm_extended_hubbard_development_amd import surrogate
+
+    rng = np.random.default_rng(0)
+    lo = np.array([4.0, 0.0, -0.3, 0.0, 0.0, 2.0, -5.0])
+    hi = np.array([10.0, 2.0, 0.3, 1.0, 1.0, 20.0, 15.0])
+    x = lo + (hi - lo) * rng.random((n, 7))
+    y = np.stack([np.tanh((x[:, 6] - x[:, 0] / 2) * x[:, 5] / 10) + 1, np.exp(-((x[:, 6] - 3) / 4) ** 2)], axis=1)
+    s = surrogate.Surrogate(y_mean=(1.0, 0.5), y_scale=(1.0, 0.5), batch=B, lr=1e-3, seed=1)
+    try:
+        if eager:
+            xb, yb = x[:B], y[:B]
+            s.step(xb, yb)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.step(xb, yb)
+            dt = time.perf_counter() - t0
+        else:
+            s.set_data(x, y)
+            s.train(20, seed=1)
+            t0 = time.perf_counter()
+            s.train(steps, seed=2)
+            dt = time.perf_counter() - t0
+    finally:
+        s.close()
+    ms = dt * 1000.0 / steps
+    flop = surrogate_step_flop(B)
+    return {"mode": "eager" if eager else "graph", "batch": B, "steps": steps, "ms_per_step": round(ms, 4),
+            "GFLOP_per_step": round(flop / 1e9, 3), "TFLOPs": round(flop / ms / 1e9, 2),
+            "frac_bf16_dense_peak": round(flop / ms / 1e9 / 2500.0, 5)}
+
+
 def read_measured_sweep():
     """The whole 4x4 sweep as run by the drop-in executable (resumable
     chunks, committed summary): total wall seconds and its parts."""
@@ -304,6 +354,10 @@ def main():
     ap.add_argument("--ceiling", action="store_true",
                     help="only measure what the HBM path delivers for the apply's shapes (stream read, copy, random "
                          "2 KB-row gather of 35 rows per row) on the workload block's vector size, and print it")
+    ap.add_argument("--surrogate", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS surrogate training steps (batch 4096, replayed HIP graphs) and print them")
+    ap.add_argument("--surrogate-eager", action="store_true",
+                    help="with --surrogate: eager launches instead of graph replays (for kernel traces)")
     ap.add_argument("--tile", default="auto",
                     help="H-apply configuration: auto (autotuner) or PATH,W (path 1 hop list / 2 on the fly; "
                          "W -1 automatic, 0 untiled, >0 tile width)")
@@ -314,6 +368,10 @@ def main():
         sys.exit(2)
 
     import ftlm_extended_hubbard_development_amd as ftlm
+
+    if args.surrogate:
+        print(json.dumps(surrogate_step(args.surrogate, args.surrogate_eager)), flush=True)
+        return
 
     if args.copy_calibration:
         import torch

@@ -146,6 +146,8 @@ __device__ inline bf16x8 ld8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
+inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
 // ------------------------------------------------------------------ GEMM
 enum { EPI_F32 = 0, EPI_GELU_FWD = 1, EPI_OUT = 2, EPI_GELU_BWD = 3 };
 
@@ -159,6 +161,49 @@ struct Epi {
   int ldt;
 };
 
+// acc[t] += A_rows(pa) B_rows(pb + t bstep)^T over K in steps of 32 (four
+// 16-column blocks); the next step's fragments load while this step's
+// products run.
+__device__ __forceinline__ void mma_k_loop(const uint16_t* pa, const uint16_t* pb, size_t bstep, int K, f32x4* acc) {
+  bf16x8 a = ld8(pa), b0 = ld8(pb), b1 = ld8(pb + bstep), b2 = ld8(pb + 2 * bstep), b3 = ld8(pb + 3 * bstep);
+  for (int k = 0; k < K; k += 32) {
+    bf16x8 na = a, nb0 = b0, nb1 = b1, nb2 = b2, nb3 = b3;
+    if (k + 32 < K) {
+      na = ld8(pa + k + 32);
+      nb0 = ld8(pb + k + 32);
+      nb1 = ld8(pb + bstep + k + 32);
+      nb2 = ld8(pb + 2 * bstep + k + 32);
+      nb3 = ld8(pb + 3 * bstep + k + 32);
+    }
+    acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b3, acc[3], 0, 0, 0);
+    a = na;
+    b0 = nb0;
+    b1 = nb1;
+    b2 = nb2;
+    b3 = nb3;
+  }
+}
+
+// The same for a fixed K of 32 S: every fragment is loaded before the first
+// product, so the whole K range costs one memory latency.
+template <int S>
+__device__ __forceinline__ void mma_k_fixed(const uint16_t* pa, const uint16_t* pb, size_t bstep, f32x4* acc) {
+  bf16x8 a[S], b[S][4];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    a[k] = ld8(pa + 32 * k);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) b[k][t] = ld8(pb + t * bstep + 32 * k);
+  }
+#pragma unroll
+  for (int k = 0; k < S; ++k)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k], b[k][t], acc[t], 0, 0, 0);
+}
+
 // One workgroup = 4 wavefronts = a 64-row x 64-column tile of C; wavefront w
 // owns rows 16w..16w+15 and the tile's four 16 x 16 column blocks, stepping
 // K by 32 with one v_mfma_f32_16x16x32_bf16 per column block. Operand
@@ -169,11 +214,14 @@ struct Epi {
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, int lda, const uint16_t* __restrict__ B,
                                               int ldb, int M, int N, int K, Epi e) {
+  constexpr bool kStaged = EPI == EPI_GELU_FWD || EPI == EPI_GELU_BWD;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int m0 = blockIdx.y * 64 + wave * 16;
   const int n0 = blockIdx.x * 64;
-  if (m0 >= M) return;  // whole wavefront
+  // the staged epilogues need every wavefront at the barrier
+  if (!kStaged && m0 >= M) return;  // whole wavefront
+  const bool act = m0 < M;
   const int r = lane & 15;
   const int kq = (lane >> 4) * 8;
   int nt = (N - n0) / 16;
@@ -183,49 +231,77 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t* __restrict__ A, in
   const uint16_t* pa = A + (size_t)(m0 + r) * lda + kq;
   const uint16_t* pb = B + (size_t)(n0 + r) * ldb + kq;
   const size_t bstep = (size_t)16 * ldb;
-  if (nt == 4) {
-    for (int k = 0; k < K; k += 32) {
-      const bf16x8 a = ld8(pa + k);
-      const bf16x8 b0 = ld8(pb + k);
-      const bf16x8 b1 = ld8(pb + bstep + k);
-      const bf16x8 b2 = ld8(pb + 2 * bstep + k);
-      const bf16x8 b3 = ld8(pb + 3 * bstep + k);
-      acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[1], 0, 0, 0);
-      acc[2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b2, acc[2], 0, 0, 0);
-      acc[3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b3, acc[3], 0, 0, 0);
-    }
-  } else {
-    for (int k = 0; k < K; k += 32) {
-      const bf16x8 a = ld8(pa + k);
-      for (int t = 0; t < nt; ++t) {
-        const bf16x8 b = ld8(pb + t * bstep + k);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+  if (act) {
+    if (nt == 4 && K == 256) {
+      mma_k_fixed<4>(pa, pb, bstep, acc);
+      mma_k_fixed<4>(pa + 128, pb + 128, bstep, acc);
+    } else if (nt == 4) {
+      mma_k_loop(pa, pb, bstep, K, acc);
+    } else {
+      for (int k = 0; k < K; k += 32) {
+        const bf16x8 a = ld8(pa + k);
+        for (int t = 0; t < nt; ++t) {
+          const bf16x8 b = ld8(pb + t * bstep + k);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
       }
     }
   }
   const int mr = m0 + (lane >> 4) * 4;
-  for (int t = 0; t < nt; ++t) {
-    const int n = n0 + 16 * t + r;
-    for (int i = 0; i < 4; ++i) {
-      const int m = mr + i;
-      const float v = acc[t][i];
-      if (EPI == EPI_F32) {
-        e.zf[(size_t)m * e.ldz + n] = v;
-      } else if (EPI == EPI_OUT) {
-        e.zf[(size_t)m * e.ldz + n] = v + e.bias[n];
-      } else if (EPI == EPI_GELU_FWD) {
-        const float z = v + e.bias[n];
-        if (e.zf) e.zf[(size_t)m * e.ldz + n] = z;
-        const uint16_t h = f2bf(gelu(z));
-        e.hb[(size_t)m * e.ldh + n] = h;
-        if (e.ht) e.ht[(size_t)n * e.ldt + m] = h;
-      } else {  // EPI_GELU_BWD
-        const float d = v * dgelu(e.zf[(size_t)m * e.ldz + n]);
-        const uint16_t h = f2bf(d);
-        e.hb[(size_t)m * e.ldh + n] = h;
-        if (e.ht) e.ht[(size_t)n * e.ldt + m] = h;
+  if (!kStaged) {
+    for (int t = 0; t < nt; ++t) {
+      const int n = n0 + 16 * t + r;
+      for (int i = 0; i < 4; ++i) {
+        const int m = mr + i;
+        const float v = acc[t][i];
+        if (EPI == EPI_F32) e.zf[(size_t)m * e.ldz + n] = v;
+        else e.zf[(size_t)m * e.ldz + n] = v + e.bias[n];  // EPI_OUT
       }
+    }
+    return;
+  }
+  // GELU epilogues: the tile's bf16 results go through LDS so that both the
+  // row-major and the transposed copy are written as 16-byte runs
+  __shared__ uint16_t th[64][72];
+  if (act) {
+    for (int t = 0; t < nt; ++t) {
+      const int n = n0 + 16 * t + r;
+      for (int i = 0; i < 4; ++i) {
+        const int m = mr + i;
+        const float v = acc[t][i];
+        uint16_t h;
+        if (EPI == EPI_GELU_FWD) {
+          const float z = v + e.bias[n];
+          if (e.zf) e.zf[(size_t)m * e.ldz + n] = z;
+          h = f2bf(gelu(z));
+        } else {
+          h = f2bf(v * dgelu(e.zf[(size_t)m * e.ldz + n]));
+        }
+        th[m - blockIdx.y * 64][16 * t + r] = h;
+      }
+    }
+  }
+  __syncthreads();
+  const int mrows = M - (int)blockIdx.y * 64 < 64 ? M - (int)blockIdx.y * 64 : 64;
+  const int ncols = nt * 16;
+  for (int c = threadIdx.x; c < 64 * 8; c += 256) {
+    const int ml = c >> 3, nc = (c & 7) * 8;
+    if (ml < mrows && nc < ncols) {
+      uint32_t w[4];
+      for (int j = 0; j < 4; ++j) w[j] = (uint32_t)th[ml][nc + 2 * j] | ((uint32_t)th[ml][nc + 2 * j + 1] << 16);
+      *reinterpret_cast<uint4*>(e.hb + (size_t)(blockIdx.y * 64 + ml) * e.ldh + n0 + nc) =
+          make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+  if (!e.ht) return;
+  for (int c = threadIdx.x; c < 64 * 8; c += 256) {
+    const int nl = c >> 3, mc = (c & 7) * 8;
+    if (nl < ncols && mc < mrows) {
+      uint32_t w[4];
+      for (int j = 0; j < 4; ++j)
+        w[j] = (uint32_t)th[mc + 2 * j][nl] | ((uint32_t)th[mc + 2 * j + 1][nl] << 16);
+      *reinterpret_cast<uint4*>(e.ht + (size_t)(n0 + nl) * e.ldt + blockIdx.y * 64 + mc) =
+          make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
 }
@@ -237,6 +313,94 @@ void gemm(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, 
     throw Error(FTLM_ERR_ARG, "MLP GEMM shape not supported");
   dim3 grid((N + 63) / 64, (M + 63) / 64);
   hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(256), 0, s, A, lda, B, ldb, M, N, K, e);
+  MLPCHECK(hipGetLastError());
+}
+
+// Weight-gradient products C = A B^T with a long K (the batch): the same
+// 64 x 64 tile per workgroup, K cut into gridDim.z chunks of KC; each
+// workgroup writes its fp32 partial tile to ws[z] (M x N), and
+// k_splitk_sum adds the chunks in increasing z (the same bits run to run).
+template <int KC>
+__global__ __launch_bounds__(256) void k_gemm_splitk(const uint16_t* __restrict__ A, int lda,
+                                                     const uint16_t* __restrict__ B, int ldb, int M, int N,
+                                                     float* __restrict__ ws) {
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int m0 = blockIdx.y * 64 + wave * 16;
+  const int n0 = blockIdx.x * 64;
+  const int z = blockIdx.z;
+  if (m0 >= M) return;  // whole wavefront
+  const int r = lane & 15;
+  const int kq = (lane >> 4) * 8;
+  int nt = (N - n0) / 16;
+  if (nt > 4) nt = 4;
+  float* part = ws + (size_t)z * M * N;
+  f32x4 acc[4];
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* pa = A + (size_t)(m0 + r) * lda + (size_t)z * KC + kq;
+  const uint16_t* pb = B + (size_t)(n0 + r) * ldb + (size_t)z * KC + kq;
+  const size_t bstep = (size_t)16 * ldb;
+  if (nt == 4) {
+    static_assert(KC % 128 == 0, "split-K chunks are multiples of 128");
+#pragma unroll
+    for (int h = 0; h < KC; h += 128) mma_k_fixed<4>(pa + h, pb + h, bstep, acc);
+  } else {
+    for (int k = 0; k < KC; k += 32) {
+      const bf16x8 a = ld8(pa + k);
+      for (int t = 0; t < nt; ++t) {
+        const bf16x8 b = ld8(pb + t * bstep + k);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int mr = m0 + (lane >> 4) * 4;
+  for (int t = 0; t < nt; ++t) {
+    const int n = n0 + 16 * t + r;
+    for (int i = 0; i < 4; ++i) part[(size_t)(mr + i) * N + n] = acc[t][i];
+  }
+}
+
+// C[m][n] = sum_z ws[z][m][n], z increasing; one output per thread, the
+// chunk values of a group of 8 loaded before they are added.
+__global__ __launch_bounds__(256) void k_splitk_sum(const float* __restrict__ ws, int S, int M, int N,
+                                                    float* __restrict__ C, int ldc) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)M * N) return;
+  const int m = (int)(i / N), n = (int)(i % N);
+  const size_t stride = (size_t)M * N;
+  float v = 0.f;
+  int q = 0;
+  for (; q + 8 <= S; q += 8) {
+    float a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = ws[(size_t)(q + u) * stride + i];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += a[u];
+  }
+  for (; q < S; ++q) v += ws[(size_t)q * stride + i];
+  C[(size_t)m * ldc + n] = v;
+}
+
+constexpr int kSplitK = 256;     // K per chunk of the split-K weight gradients
+constexpr int kLossCounter = 0;  // the loss's arrival counter in wg_cnt
+
+// fp32 C = A B^T for a weight gradient; K split in chunks of kSplitK when it
+// divides K into more than one chunk (ws: K / kSplitK * M * N floats), else
+// the one-pass kernel.
+void gemm_wgrad(const uint16_t* A, int lda, const uint16_t* B, int ldb, int M, int N, int K, float* C, int ldc,
+                float* ws, hipStream_t s) {
+  if (K % kSplitK != 0 || K / kSplitK < 2) {
+    Epi e{};
+    e.zf = C;
+    e.ldz = ldc;
+    gemm<EPI_F32>(A, lda, B, ldb, M, N, K, e, s);
+    return;
+  }
+  if (M % 16 || N % 16 || lda % 8 || ldb % 8 || M <= 0 || N <= 0 || (size_t)M * N > (size_t)kHid * kHid)
+    throw Error(FTLM_ERR_ARG, "MLP GEMM shape not supported");
+  dim3 grid((N + 63) / 64, (M + 63) / 64, K / kSplitK);
+  hipLaunchKernelGGL(k_gemm_splitk<kSplitK>, grid, dim3(256), 0, s, A, lda, B, ldb, M, N, ws);
+  hipLaunchKernelGGL(k_splitk_sum, dim3(blocks_for((int64_t)M * N)), dim3(256), 0, s, ws, K / kSplitK, M, N, C, ldc);
   MLPCHECK(hipGetLastError());
 }
 
@@ -271,20 +435,25 @@ __global__ void k_gather(const uint16_t* __restrict__ feat, const float* __restr
 
 // Mean-squared error over n_valid rows and the 2 outputs, L = sum e^2 / (2 n);
 // dL/dY = e / n as bf16 in both layouts (padding rows and columns 0). One
-// workgroup; fixed reduction order.
+// element per thread; each workgroup sums its squares in a fixed tree, the
+// last workgroup to finish (arrival counter, reset for the next launch) adds
+// the workgroups' sums in a fixed order (thread t takes workgroups t, t + 256,
+// ..., then a tree over the threads).
 __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, const float* __restrict__ T, int rows,
                                               int n_valid, uint16_t* __restrict__ dY, uint16_t* __restrict__ dYt,
-                                              float* __restrict__ loss, const int32_t* __restrict__ iter) {
+                                              float* __restrict__ loss, const int32_t* __restrict__ iter,
+                                              float* __restrict__ part, unsigned* __restrict__ cnt) {
   __shared__ float red[256];
   const float inv_n = 1.0f / (float)n_valid;
+  const int i = blockIdx.x * 256 + threadIdx.x;
   float acc = 0.f;
-  for (int i = threadIdx.x; i < rows * kOutPad; i += 256) {
+  if (i < rows * kOutPad) {
     const int b = i / kOutPad;
     const int c = i % kOutPad;
     float d = 0.f;
     if (b < n_valid && c < 2) {
       const float err = Y[i] - T[b * 2 + c];
-      acc += err * err;
+      acc = err * err;
       d = err * inv_n;
     }
     const uint16_t h = f2bf(d);
@@ -297,11 +466,32 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ Y, const
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss[iter ? *iter : 0] = 0.5f * red[0] * inv_n;
+  __shared__ unsigned last;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0];
+    __threadfence();
+    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  float sum = 0.f;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += 256) sum += __builtin_nontemporal_load(&part[b]);
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    loss[iter ? *iter : 0] = 0.5f * red[0] * inv_n;
+    *cnt = 0u;
+  }
 }
 
-// out[r] = sum_c M[r][c] (bf16 rows of length cols), one wavefront per row,
-// fixed order: lane-strided partial sums, then a butterfly.
+// out[r] = sum_c M[r][c] (bf16 rows of length cols, cols % 8 == 0), one
+// wavefront per row, fixed order: each lane adds runs of 8 values (one
+// 16-byte load) strided by 512 columns, then a butterfly over the lanes.
 __global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ Mt, int cols, int nrows,
                                                 float* __restrict__ out) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -309,7 +499,15 @@ __global__ __launch_bounds__(256) void k_rowsum(const uint16_t* __restrict__ Mt,
   if (row >= nrows) return;
   const uint16_t* p = Mt + (size_t)row * cols;
   float s = 0.f;
-  for (int c = lane; c < cols; c += 64) s += bf2f(p[c]);
+  for (int c = 8 * lane; c < cols; c += 512) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p + c);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      s += bf2f((uint16_t)(w[j] & 0xffffu));
+      s += bf2f((uint16_t)(w[j] >> 16));
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (lane == 0) out[row] = s;
 }
@@ -393,7 +591,6 @@ __global__ void k_zero_f32(float* p, int64_t n) {
   if (i < n) p[i] = 0.f;
 }
 
-inline unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // ------------------------------------------------------ host encoding
 constexpr double kTwoPi = 6.283185307179586476925286766559;
@@ -435,6 +632,8 @@ struct ftlm_mlp {
   Buf x0, x0t, h[kNHid + 1], ht[kNHid + 1], z[kNHid], y, t, dy, dyt, dz[2], dzt[2];
   Buf feat_host_dev;  // fp32 feature staging
   Buf data_feat, data_tgt, idx, loss, counters;
+  Buf wg_ws, wg_cnt;  // split-K partial tiles; the loss's arrival counter
+  Buf loss_part;      // per-workgroup loss sums
   int64_t n_data = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<uint64_t> exec_key;  // what the graph baked in (buffers, batch, rate)
@@ -470,6 +669,12 @@ struct ftlm_mlp {
       dz[i].get<uint16_t>(rows * kHid);
       dzt[i].get<uint16_t>(rows * kHid);
     }
+    wg_ws.get<float>((size_t)(rows / kSplitK + 1) * kHid * kHid);
+    loss_part.get<float>((size_t)(rows * kOutPad + 255) / 256);
+    if (!wg_cnt.p) {
+      MLPCHECK(hipMemset(wg_cnt.get<unsigned>(64), 0, 64 * sizeof(unsigned)));
+      MLPCHECK(hipDeviceSynchronize());
+    }
     cap = rows;
   }
 
@@ -500,16 +705,15 @@ struct ftlm_mlp {
 
   // loss + backward into grads (rows with index >= n_valid contribute 0)
   void backward(int rows, int n_valid, float* loss_out, const int32_t* iter) {
-    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, stream, y.as<float>(), t.as<float>(), rows, n_valid,
-                       dy.as<uint16_t>(), dyt.as<uint16_t>(), loss_out, iter);
+    hipLaunchKernelGGL(k_loss, dim3(blocks_for((int64_t)rows * kOutPad)), dim3(256), 0, stream, y.as<float>(),
+                       t.as<float>(), rows, n_valid, dy.as<uint16_t>(), dyt.as<uint16_t>(), loss_out, iter,
+                       loss_part.as<float>(), wg_cnt.as<unsigned>() + kLossCounter);
     MLPCHECK(hipGetLastError());
     float* g = grads.as<float>();
     // output layer: dW_4 = dY^T H_4, db_4 = colsum dY, dH_4 = dY W_4 -> dZ_3
     {
-      Epi e{};
-      e.zf = g + w_off(kNHid);
-      e.ldz = kHid;
-      gemm<EPI_F32>(dyt.as<uint16_t>(), rows, ht[kNHid].as<uint16_t>(), rows, kOutPad, kHid, rows, e, stream);
+      gemm_wgrad(dyt.as<uint16_t>(), rows, ht[kNHid].as<uint16_t>(), rows, kOutPad, kHid, rows, g + w_off(kNHid),
+                 kHid, wg_ws.as<float>(), stream);
       hipLaunchKernelGGL(k_rowsum, dim3(blocks_for(kOutPad, 4)), dim3(256), 0, stream, dyt.as<uint16_t>(), rows,
                          kOutPad, g + b_off(kNHid));
       MLPCHECK(hipGetLastError());
@@ -527,10 +731,8 @@ struct ftlm_mlp {
     for (int j = kNHid - 1; j >= 0; --j) {
       const int kin = layer_in(j);
       const uint16_t* hin_t = j == 0 ? x0t.as<uint16_t>() : ht[j].as<uint16_t>();
-      Epi e{};
-      e.zf = g + w_off(j);
-      e.ldz = kin;
-      gemm<EPI_F32>(dzt[cur].as<uint16_t>(), rows, hin_t, rows, kHid, kin, rows, e, stream);
+      gemm_wgrad(dzt[cur].as<uint16_t>(), rows, hin_t, rows, kHid, kin, rows, g + w_off(j), kin, wg_ws.as<float>(),
+                 stream);
       hipLaunchKernelGGL(k_rowsum, dim3(blocks_for(kHid, 4)), dim3(256), 0, stream, dzt[cur].as<uint16_t>(), rows,
                          kHid, g + b_off(j));
       MLPCHECK(hipGetLastError());
@@ -862,10 +1064,16 @@ int ftlm_mlp_gemm(ftlm_mlp* m, int M, int N, int K, const float* a, const float*
                        (int64_t)N * K / kFeat, (int64_t)N * K / kFeat, hb, (uint16_t*)nullptr);
     MLPCHECK(hipGetLastError());
     float* dc = fc.get<float>((size_t)M * N);
-    Epi e{};
-    e.zf = dc;
-    e.ldz = N;
-    gemm<EPI_F32>(ha, K, hb, K, M, N, K, e, m->stream);
+    if (K % kSplitK == 0 && K / kSplitK >= 2 && (size_t)M * N <= (size_t)kHid * kHid) {
+      // the split-K weight-gradient kernel (what training runs for K = batch)
+      Buf ws;
+      gemm_wgrad(ha, K, hb, K, M, N, K, dc, N, ws.get<float>((size_t)(K / kSplitK) * M * N), m->stream);
+    } else {
+      Epi e{};
+      e.zf = dc;
+      e.ldz = N;
+      gemm<EPI_F32>(ha, K, hb, K, M, N, K, e, m->stream);
+    }
     MLPCHECK(hipMemcpyAsync(c, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost, m->stream));
     m->sync();
   });

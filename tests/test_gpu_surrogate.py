@@ -1,9 +1,11 @@
 """GPU surrogate kernels (ftlm_mlp_* of libftlm_hip.so) against the CPU
 oracle oracle/mlp_oracle.py.
 
-* The MFMA GEMM kernel against float64 products of the same bf16-rounded
-  operands: the products of two bf16 numbers are exact in fp32, so the only
-  difference is fp32 accumulation (relative error ~ K * 2^-24).
+* The MFMA GEMM kernels (one pass, and split-K for K a multiple of 256 with
+  two or more chunks: the weight gradients of a training batch) against
+  float64 products of the same bf16-rounded operands: the products of two
+  bf16 numbers are exact in fp32, so the only difference is fp32
+  accumulation (relative error ~ K * 2^-24).
 * Forward pass and gradients against the oracle with bf16 rounding emulated
   at the points where the kernels store bf16. What remains is fp32 versus
   float64 accumulation, which now and then moves a stored bf16 value by one
@@ -46,7 +48,8 @@ def rand_inputs(n, seed):
     return lo + (hi - lo) * rng.random((n, 7))
 
 
-@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (64, 64, 32), (128, 256, 256), (48, 80, 96), (256, 32, 4096)])
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (64, 64, 32), (128, 256, 256), (48, 80, 96), (256, 32, 4096),
+                                   (256, 256, 4096), (32, 256, 512), (80, 48, 1024)])
 def test_mfma_gemm(sur, M, N, K):
     say(f"gemm M={M} N={N} K={K}")
     rng = np.random.default_rng(M * 7 + N * 3 + K)
@@ -90,7 +93,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
-@pytest.mark.parametrize("n", [256, 100])
+@pytest.mark.parametrize("n", [256, 100, 4096, 1000])
 def test_gradients_match_oracle(sur, n):
     p = sur.params().astype(np.float64)
     x = rand_inputs(n, 4)
