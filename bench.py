@@ -433,6 +433,9 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
     achieved = bytes_apply / (ms_apply / 1000.0) / 1e9
+    place_ms, place_pick = eng.vector_placement_report()
+    print(f"# vector placement: apply ms per candidate buffer {[round(float(v), 2) for v in place_ms]}, "
+          f"kept {place_pick}", file=sys.stderr, flush=True)
 
     # plain per-vector GPU FTLM on the same block (R = 1 kernels)
     eng.prepare(R, bseed)
@@ -482,7 +485,9 @@ def main():
                                f"(GPU start vectors, tree reductions, QL Ritz)",
                    "lattice": "4x4", "u": 7.0, "R": R, "lanczos_steps": m, "block_dim": dim,
                    "apply_config": {"path": ["auto", "hoplist", "onthefly"][tile[0]], "tile": tile[1]},
-                   "autotune_ms_per_apply": tune},
+                   "autotune_ms_per_apply": tune,
+                   "vector_placement": {"apply_ms_per_candidate": [round(float(v), 2) for v in place_ms],
+                                        "kept": place_pick}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                      "algorithmic_bytes_per_launch": per_launch,

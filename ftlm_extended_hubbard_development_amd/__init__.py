@@ -72,6 +72,8 @@ SIGNATURES = {
     "ftlm_set_apply_config": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "ftlm_get_apply_config": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip]),
     "ftlm_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp]),
+    "ftlm_set_vector_placement": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
+    "ftlm_vector_placement_report": (C.c_int, [C.c_void_p, _ip, _dp, _ip]),
     "ftlm_set_twist": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "ftlm_pack_begin": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_uint64]),
     "ftlm_pack_add": (C.c_int, [C.c_void_p, C.c_uint64, _lp]),
@@ -286,6 +288,18 @@ class Engine:
         w = C.c_int32(0)
         self._check(self.lib.ftlm_get_apply_config(self.h, R, C.byref(p), C.byref(w)))
         return p.value, w.value
+
+    def set_vector_placement(self, candidates=8, min_bytes=256 << 20):
+        """Placement tuning of the Lanczos vectors (0 candidates: off)."""
+        self._check(self.lib.ftlm_set_vector_placement(self.h, candidates, min_bytes))
+
+    def vector_placement_report(self):
+        """(apply ms per candidate of the last tuning, chosen candidates)."""
+        n = C.c_int32(16)
+        ms = np.zeros(16)
+        pick = np.zeros(3, np.int32)
+        self._check(self.lib.ftlm_vector_placement_report(self.h, C.byref(n), _p(ms, _dp), _p(pick, _ip)))
+        return ms[:n.value].copy(), (pick.tolist() if n.value else [])
 
     def autotune(self, R):
         """Time the apply candidates on the current block; returns

@@ -97,6 +97,10 @@ struct DevBuf {
   T* ptr() const {
     return static_cast<T*>(p);
   }
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
 };
 
 constexpr double kPi = 3.141592653589793238462643383279502884;
@@ -257,6 +261,15 @@ struct ftlm_ctx {
   DevBuf vec[3], part, counter, st_alpha, st_beta, st_scale, st_sprev, st_bprev, st_dot, st_nrm, st_done, st_mcount,
       ritz_eig, ritz_w, ritz_scratch, ritz_iscratch, ritz_fail, pvec[3], pristine;
   int last_reduce = FTLM_REDUCE_TREE;
+  // placement of the Lanczos vectors: when they are (re)allocated at
+  // place_min_bytes or more, the next batched run times the hop-list apply
+  // from place_candidates buffers and keeps the fastest two as the gather
+  // sources (0 candidates: off)
+  int place_candidates = 8;
+  uint64_t place_min_bytes = (uint64_t)256 << 20;
+  bool place_pending = false;
+  std::vector<double> place_ms;  // last tuning: apply ms per candidate
+  std::vector<int> place_pick;   // candidates chosen for vec[0], vec[1], vec[2]
   int prepared_R = 0;
   int prepared_layout = -1;  // 0: [dim][R], 1: [R][dim]
   // H-apply configuration: path 0 automatic (hop list), 1 hop list, 2 on the
@@ -630,7 +643,12 @@ void gen_starts(uint64_t bseed, int r0, int cnt, int64_t dim, double* host) {
 // device vector buffers for R columns of the current block
 void ensure_vectors(ftlm_ctx* c, int R) {
   const size_t bytes = (size_t)c->cur.dim * R * 16;
-  for (int i = 0; i < 3; ++i) c->vec[i].get(bytes);
+  bool grown = false;
+  for (int i = 0; i < 3; ++i) {
+    grown = grown || bytes > c->vec[i].bytes;
+    c->vec[i].get(bytes);
+  }
+  if (grown && c->place_candidates >= 3 && bytes >= c->place_min_bytes) c->place_pending = true;
 }
 
 // upload seeded start vectors of samples sample0..sample0+R-1 into layout 0
@@ -913,6 +931,102 @@ void lanczos_run(ftlm_ctx* c, int R, int steps, int reduce, double* x0, double* 
   }
 }
 
+// Placement of the Lanczos vectors. The batched hop-list gather reads ~35
+// random rows of x per row of y, and how fast the HBM path serves them
+// depends on where x physically lies: the same apply on the same data takes
+// from ~117 to ~132 ms on the 4x4 (8,8) block depending on the buffer x is
+// in, reproducibly for that buffer, and the output buffer moves it too.
+// Called once after the vectors were (re)allocated: time one apply from each
+// of the three vector buffers and up to place_candidates - 3 extra ones
+// (what free memory allows, minus a reserve), keep the fastest two as vec[0]
+// and vec[1] (the buffers the recurrence alternates as the gather source),
+// then time the applies from both into each remaining candidate and keep the
+// fastest as vec[2] (the output); free the rest. vec[0] holds the prepared start vectors; they are
+// copied into the chosen buffer. Results do not depend on the choice.
+void tune_placement(ftlm_ctx* c, int R) {
+  c->place_pending = false;
+  const ApplyCfg cfg = resolve_cfg(c, R, c->apply_path, c->apply_w);
+  if (cfg.path != 1 || cfg.w != 0 || R < 2 || c->cur.dim == 0) return;
+  const int64_t dim = c->cur.dim;
+  const size_t need = (size_t)dim * R * 16;
+  const size_t cap = c->vec[0].bytes;
+  if (c->vec[1].bytes != cap || c->vec[2].bytes != cap || cap < need) return;
+  const HopList& h = ensure_hops(c);
+  size_t fr = 0, tot = 0;
+  HIPCHECK(hipMemGetInfo(&fr, &tot));
+  const size_t reserve = std::max<size_t>((size_t)4 << 30, tot / 20);
+  std::vector<std::unique_ptr<DevBuf>> cand;
+  for (int i = 0; i < 3; ++i) {
+    cand.push_back(std::make_unique<DevBuf>());
+    cand.back()->swap(c->vec[i]);
+  }
+  for (int i = 3; i < c->place_candidates; ++i) {
+    if (fr < reserve + cap) break;
+    void* p = nullptr;
+    if (hipMalloc(&p, cap) != hipSuccess) {
+      (void)hipGetLastError();
+      break;
+    }
+    cand.push_back(std::make_unique<DevBuf>());
+    cand.back()->p = p;
+    cand.back()->bytes = cap;
+    fr -= cap;
+  }
+  const int n = (int)cand.size();
+  hipStream_t s = c->stream;
+  // finite contents in every buffer but the start vectors (apply input only)
+  for (int i = 1; i < n; ++i) launch_zero(cand[i]->ptr<double>(), (int64_t)(need / 8), s);
+  const int64_t* offs = c->cur.offs.ptr<int64_t>();
+  auto apply = [&](int xi, int yi) {
+    launch_apply_hl(h, dim, R, 0, offs, c->n_canon, c->cus, cand[xi]->ptr<double>(), cand[yi]->ptr<double>(),
+                    nullptr, nullptr, nullptr, s);
+  };
+  apply(0, 1);  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  auto timed = [&](int xi, int yi) {
+    HIPCHECK(hipEventRecord(e0, s));
+    apply(xi, yi);
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    return (double)t;
+  };
+  // the gather sources: each candidate as x (never writing cand[0], which
+  // holds the start vectors)
+  std::vector<double> ms(n);
+  for (int i = 0; i < n; ++i) ms[i] = timed(i, i == 1 ? 2 : 1);
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ms[a] < ms[b]; });
+  // the output: the remaining candidate with the least time of the applies
+  // from both sources into it
+  int out = -1;
+  double best = 1e300;
+  for (int k = 2; k < n; ++k) {
+    const int j = order[k];
+    if (j == 0) continue;  // keep the start vectors until they are copied
+    const double t = timed(order[0], j) + timed(order[1], j);
+    if (t < best) {
+      best = t;
+      out = j;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHECK(hipGetLastError());
+  if (out < 0) out = order[2];
+  if (order[0] != 0)
+    HIPCHECK(hipMemcpyAsync(cand[order[0]]->p, cand[0]->p, need, hipMemcpyDeviceToDevice, s));
+  sync(c);
+  const int pick[3] = {order[0], order[1], out};
+  for (int i = 0; i < 3; ++i) c->vec[i].swap(*cand[pick[i]]);
+  c->place_ms = ms;
+  c->place_pick.assign(pick, pick + 3);
+}
+
 // Lanczos of R columns stored in vec[0] (layout 0) or, per vector, of the
 // sample-major columns of vec[0] (layout 1). Leaves alpha/beta/mcount of all
 // R columns in the ctx state arrays ([steps][R]).
@@ -920,6 +1034,7 @@ void run_all(ftlm_ctx* c, int R, int steps, int reduce, bool per_vector, bool ti
   const int64_t dim = c->cur.dim;
   LanczosState st = lanczos_state(c, R, steps);
   if (!per_vector) {
+    if (c->place_pending) tune_placement(c, R);
     lanczos_run(c, R, steps, reduce, c->vec[0].ptr<double>(), c->vec[1].ptr<double>(), c->vec[2].ptr<double>(), st,
                 false, timed);
     return;
@@ -2058,6 +2173,29 @@ int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W) {
     const ApplyCfg a = resolve_cfg(ctx, R, ctx->apply_path, ctx->apply_w);
     *path = a.path;
     *W = a.w;
+  });
+}
+
+int ftlm_set_vector_placement(ftlm_ctx* ctx, int candidates, uint64_t min_bytes) {
+  return guarded([&] {
+    if (!ctx) throw Error(FTLM_ERR_ARG, "null context");
+    if (candidates != 0 && (candidates < 3 || candidates > 16))
+      throw Error(FTLM_ERR_ARG, "placement candidates must be 0 (off) or 3 to 16");
+    ctx->place_candidates = candidates;
+    ctx->place_min_bytes = min_bytes;
+    if (candidates == 0) ctx->place_pending = false;
+  });
+}
+
+int ftlm_vector_placement_report(ftlm_ctx* ctx, int* n, double* ms, int* pick) {
+  return guarded([&] {
+    if (!ctx || !n) throw Error(FTLM_ERR_ARG, "null argument");
+    const int have = (int)ctx->place_ms.size();
+    if (ms)
+      for (int i = 0; i < std::min(*n, have); ++i) ms[i] = ctx->place_ms[i];
+    if (pick)
+      for (int i = 0; i < (int)ctx->place_pick.size(); ++i) pick[i] = ctx->place_pick[i];
+    *n = have;
   });
 }
 

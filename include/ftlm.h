@@ -153,6 +153,24 @@ int ftlm_last_stats(ftlm_ctx* ctx, ftlm_stats* out);
 int ftlm_set_apply_config(ftlm_ctx* ctx, int path, int W);
 int ftlm_get_apply_config(ftlm_ctx* ctx, int R, int* path, int* W);
 
+/* Placement of the Lanczos vectors (no reference counterpart; an engine
+ * tuning knob). The batched hop-list gather's speed depends on where in HBM
+ * the gathered vector physically lies (up to ~13 % between buffers of one
+ * process on the 4x4 (8,8) block). When the three R-column vectors are
+ * (re)allocated at min_bytes or more, the next batched Lanczos run times one
+ * apply from `candidates` buffers (the three plus extra ones that free memory
+ * allows, minus a reserve of max(4 GiB, 5 % of the device)), keeps the
+ * fastest two as the buffers the recurrence alternates as the gather source,
+ * then the remaining buffer into which applies from those two are fastest as
+ * the output, and frees the rest. candidates = 0 turns it off, else 3..16 (default 8,
+ * min_bytes 256 MiB). Results do not depend on the placement.
+ * ftlm_vector_placement_report: the last tuning's apply ms per candidate as
+ * the gather source (on input *n = capacity of ms; on output the number of
+ * candidates, 0 if none ran) and, when pick is not NULL, the 3 chosen
+ * candidates (the two gather sources, then the output). */
+int ftlm_set_vector_placement(ftlm_ctx* ctx, int candidates, uint64_t min_bytes);
+int ftlm_vector_placement_report(ftlm_ctx* ctx, int* n, double* ms, int* pick);
+
 /* Autotuner: times the candidate configurations (hop list untiled and with
  * tiles 64/32/16/8 below R, on the fly untiled) on the current block with R
  * vectors and keeps the fastest. On input *ncand is the capacity of the
