@@ -13,7 +13,9 @@
 //   6364136223846793005). One state refill updates word k from words k, k+1
 //   and k+156 (mod 312) in increasing k, so it splits into three parallel
 //   phases: k < 156 (all inputs old), 156 <= k < 311 (word k-156 new, k+1 old)
-//   and k = 311 (words 0 and 155 new).
+//   and k = 311 (words 0 and 155 new). The state is double-buffered in LDS
+//   (a refill reads one copy and writes the other), so a refill costs three
+//   barriers.
 // * canonical doubles as libstdc++'s generate_canonical<double, 53> computes
 //   them from a 64-bit engine: (double)z / 2^64, and 1 - 2^-53 if that rounds
 //   to 1.
@@ -23,7 +25,8 @@
 //   (saved) second, mult = sqrt(-2 log(r2) / r2). Element i of the vector is
 //   the i-th accepted attempt: real = x * mult, imaginary = y * mult. Attempts
 //   never straddle a refill (312 draws = 156 attempts), so each refill's
-//   accepted attempts are placed by a prefix count.
+//   accepted attempts are placed by a prefix count (wavefront ballots and the
+//   counts of the preceding wavefronts).
 // * normalisation: each element divided by the column's 2-norm.
 //
 // The arithmetic is the same IEEE double sequence (built with
@@ -79,59 +82,63 @@ __device__ inline double canonical(uint64_t z) {
 __global__ __launch_bounds__(kThreads) void k_start_vectors(uint64_t bseed, int r0, int col0, int R, int64_t dim,
                                                            int layout, double* __restrict__ out,
                                                            double* __restrict__ norms) {
-  __shared__ uint64_t mt[kN];
-  __shared__ int pref[kThreads];
+  constexpr int kWaves = kThreads / 64;
+  __shared__ uint64_t mt[2][kN];
+  __shared__ int wcnt[kWaves];
   __shared__ double red[kThreads];
   const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
   const int r = r0 + blockIdx.x;
   // seeding (sequential recurrence, one thread)
   if (t == 0) {
     uint64_t s = mix64_dev(bseed ^ (uint64_t)r);
-    mt[0] = s;
+    mt[0][0] = s;
     for (int i = 1; i < kN; ++i) {
       s = 6364136223846793005ull * (s ^ (s >> 62)) + (uint64_t)i;
-      mt[i] = s;
+      mt[0][i] = s;
     }
   }
   __syncthreads();
+  int cur = 0;  // mt[cur] holds the state
   const int c = col0 + blockIdx.x;
   const int64_t stride = layout == 0 ? (int64_t)R : 1;
   double* col = layout == 0 ? out + 2 * (int64_t)c : out + 2 * (int64_t)c * dim;
   int64_t base = 0;
   double ssq = 0.0;
   while (base < dim) {
-    // refill: phase 1 (k < 156), phase 2 (156 <= k < 311), phase 3 (k = 311)
-    uint64_t v = 0;
-    if (t < kN - kM) v = twist(mt[t], mt[t + 1], mt[t + kM]);
+    // refill from mt[cur] into mt[cur ^ 1]: phase 1 (k < 156), phase 2
+    // (156 <= k < 311), phase 3 (k = 311)
+    const uint64_t* o = mt[cur];
+    uint64_t* nw = mt[cur ^ 1];
+    if (t < kN - kM) nw[t] = twist(o[t], o[t + 1], o[t + kM]);
     __syncthreads();
-    if (t < kN - kM) mt[t] = v;
+    if (t >= kN - kM && t < kN - 1) nw[t] = twist(o[t], o[t + 1], nw[t - (kN - kM)]);
     __syncthreads();
-    if (t >= kN - kM && t < kN - 1) v = twist(mt[t], mt[t + 1], mt[t - (kN - kM)]);
+    if (t == 0) nw[kN - 1] = twist(o[kN - 1], nw[0], nw[kM - 1]);
     __syncthreads();
-    if (t >= kN - kM && t < kN - 1) mt[t] = v;
-    __syncthreads();
-    if (t == 0) mt[kN - 1] = twist(mt[kN - 1], mt[0], mt[kM - 1]);
-    __syncthreads();
+    cur ^= 1;
     // attempt t (t < 156) takes draws 2t, 2t+1 of this refill
     int acc = 0;
     double x = 0.0, y = 0.0, r2 = 0.0;
     if (t < kN / 2) {
-      x = 2.0 * canonical(temper(mt[2 * t])) - 1.0;
-      y = 2.0 * canonical(temper(mt[2 * t + 1])) - 1.0;
+      x = 2.0 * canonical(temper(nw[2 * t])) - 1.0;
+      y = 2.0 * canonical(temper(nw[2 * t + 1])) - 1.0;
       r2 = x * x + y * y;
       acc = (r2 > 1.0 || r2 == 0.0) ? 0 : 1;
     }
-    // inclusive prefix count of accepted attempts (Hillis-Steele in LDS)
-    pref[t] = acc;
+    // rank of each accepted attempt: accepted attempts of the earlier
+    // wavefronts plus those of lower lanes in its own (attempt order = t)
+    const unsigned long long bal = __ballot(acc);
+    if (lane == 0) wcnt[wv] = __popcll(bal);
     __syncthreads();
-    for (int off = 1; off < kThreads; off <<= 1) {
-      const int add = t >= off ? pref[t - off] : 0;
-      __syncthreads();
-      pref[t] += add;
-      __syncthreads();
+    int before = __popcll(bal & ((1ull << lane) - 1ull));
+    int total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      if (w < wv) before += wcnt[w];
+      total += wcnt[w];
     }
     if (acc) {
-      const int64_t i = base + pref[t] - 1;
+      const int64_t i = base + before;
       if (i < dim) {
         const double mult = sqrt(-2.0 * log(r2) / r2);
         const double re = x * mult;
@@ -141,8 +148,9 @@ __global__ __launch_bounds__(kThreads) void k_start_vectors(uint64_t bseed, int 
         ssq += re * re + im * im;
       }
     }
-    base += pref[kThreads - 1];
-    __syncthreads();  // mt and pref are rewritten by the next refill
+    base += total;
+    // the next refill writes the other state copy and, after three
+    // barriers, wcnt: no barrier needed here
   }
   // column norm: thread partials, then a fixed-order tree
   red[t] = ssq;
