@@ -111,6 +111,8 @@ void print_help(const char* argv0) {
             << "                        resume from it (a few KB; the CSV is written when all are done)\n"
             << "  --stop-after S        stop between sectors before S seconds of wall time are exceeded\n"
             << "                        (predicted from this run's rate); rerun the command to continue\n"
+            << "  --vector-placement N  buffers timed for the Lanczos vectors when they are allocated\n"
+            << "                        (default 8; 0 keeps the first allocation)\n"
             << "  --help                show this message\n";
 }
 
@@ -169,6 +171,7 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
     else if (arg == "--twist-grid") p.twist_grid = parse_as<int>(next(arg));
     else if (arg == "--partial-sums") p.partial_sums = next(arg);
     else if (arg == "--stop-after") p.stop_after = parse_as<double>(next(arg));
+    else if (arg == "--vector-placement") p.vector_placement = parse_as<int>(next(arg));
     else if (arg == "--twist-list") {
       std::stringstream ss(next(arg));
       std::string item;
@@ -197,6 +200,8 @@ bool parse_options(int argc, char** argv, Options& p, bool ed_tool) {
     die("--pack-width must be 4, 8, 16, 32 or 64.");
   if (p.pack_wg < 0 || p.pack_budget_gb < 0.0) die("--pack-wg and --pack-budget-gb must be non-negative.");
   if (p.twist_grid < 0) die("--twist-grid must be non-negative.");
+  if (p.vector_placement != 0 && (p.vector_placement < 3 || p.vector_placement > 16))
+    die("--vector-placement must be 0 or 3 to 16.");
   if (p.twist_grid > 0 && !p.twist_list.empty()) die("--twist-grid and --twist-list are exclusive.");
   if (p.stop_after < 0.0) die("--stop-after must be non-negative.");
   if (p.stop_after > 0.0 && p.partial_sums.empty() && p.checkpoint.empty())

@@ -41,6 +41,9 @@ struct Options {
   // start vectors drawn on host threads even with tree reductions (which
   // otherwise draw the same stream on the GPU)
   bool host_starts = false;
+  // buffers timed for the Lanczos vectors when they are (re)allocated (the
+  // engine's placement tuning; 0 off)
+  int vector_placement = 8;
   // twist scan over listed (phix, phiy) pairs, one CSV per twist
   std::vector<std::pair<double, double>> twist_list;
   // resumable runs: the grand-canonical sums of finished sectors kept in a

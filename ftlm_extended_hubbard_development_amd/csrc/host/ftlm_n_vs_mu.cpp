@@ -226,6 +226,7 @@ int main(int argc, char** argv) {
     // serial reductions reproduce the reference bit for bit: host-drawn starts
     check(ftlm_set_start_source(ctx.c, device_thermo && !o.host_starts ? FTLM_START_DEVICE : FTLM_START_HOST),
           "start vectors");
+    check(ftlm_set_vector_placement(ctx.c, o.vector_placement, (std::uint64_t)256 << 20), "vector placement");
     int cur_tw = 0;
     auto use_twist = [&](int t) {
       if (t == cur_tw) return;

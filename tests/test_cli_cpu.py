@@ -30,6 +30,7 @@ def test_help(tmp_path):
     (["--lx", "9", "--ly", "8"], "Error: This implementation supports at most 63 sites."),
     (["--bogus"], "Error: Unknown argument: --bogus"),
     (["--mu-count", "1"], "Error: --mu-count must be at least 2."),
+    (["--vector-placement", "2"], "Error: --vector-placement must be 0 or 3 to 16."),
     (["--checkpoint", "ck.bin", "--trace-partition-csv", "t.csv"],
      "Error: --checkpoint is not currently compatible with trace/debug block output."),
 ])
