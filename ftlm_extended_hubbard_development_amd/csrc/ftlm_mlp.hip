@@ -25,7 +25,10 @@
 //   data grad    dH_j = dZ_j W_j            A = dZ_j [B x N_j],  B = W_j^T [K_j x N_j]
 //   weight grad  dW_j = dZ_j^T H_j          A = dZ_j^T [N_j x B], B = H_j^T [K_j x B]
 // so activations and their gradients are stored twice ([B x n] and [n x B])
-// and the optimiser writes each bf16 weight matrix in both orientations.
+// and the optimiser writes each bf16 weight matrix in both orientations. The
+// weight gradients' K is the batch (4,096 rows): they run split-K, one
+// workgroup per 64 x 64 tile and 256-wide K chunk, with the chunks added in
+// a fixed order, so that the long K does not serialise 16 workgroups.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
