@@ -358,6 +358,9 @@ def main():
                     help="only time STEPS surrogate training steps (batch 4096, replayed HIP graphs) and print them")
     ap.add_argument("--surrogate-eager", action="store_true",
                     help="with --surrogate: eager launches instead of graph replays (for kernel traces)")
+    ap.add_argument("--vector-placement", type=int, default=None, metavar="N",
+                    help="buffers the engine times for the Lanczos vectors when it allocates them "
+                         "(default: the engine's, 8; 0 off)")
     ap.add_argument("--tile", default="auto",
                     help="H-apply configuration: auto (autotuner) or PATH,W (path 1 hop list / 2 on the fly; "
                          "W -1 automatic, 0 untiled, >0 tile width)")
@@ -388,6 +391,8 @@ def main():
 
     nup, ndn, mx, my = (int(t) for t in args.block.split(","))
     eng = ftlm.Engine(lx=4, ly=4, u=7.0)
+    if args.vector_placement is not None:
+        eng.set_vector_placement(args.vector_placement)
     if args.ceiling:
         print(json.dumps(memory_ceiling(eng, args.R)), flush=True)
         eng.close()
