@@ -1067,9 +1067,9 @@ int ftlm_mlp_gemm(ftlm_mlp* m, int M, int N, int K, const float* a, const float*
                        (int64_t)N * K / kFeat, (int64_t)N * K / kFeat, hb, (uint16_t*)nullptr);
     MLPCHECK(hipGetLastError());
     float* dc = fc.get<float>((size_t)M * N);
+    Buf ws;  // split-K partial tiles; lives until the stream is synchronised below
     if (K % kSplitK == 0 && K / kSplitK >= 2 && (size_t)M * N <= (size_t)kHid * kHid) {
       // the split-K weight-gradient kernel (what training runs for K = batch)
-      Buf ws;
       gemm_wgrad(ha, K, hb, K, M, N, K, dc, N, ws.get<float>((size_t)(K / kSplitK) * M * N), m->stream);
     } else {
       Epi e{};
