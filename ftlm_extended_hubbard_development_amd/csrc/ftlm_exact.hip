@@ -6,7 +6,10 @@
 // Householder reflections, then one lane runs implicit QL with Wilkinson
 // shifts on (d, e). Eigenvalues are returned ascending, like zheev's.
 // Both steps are backward stable, so the eigenvalues agree with zheev's to
-// a few ulps of ||A||.
+// a few ulps of ||A||. Blocks of kBigN rows or more run in a 1024-thread
+// workgroup (a second launch over the same batch), the others in 256 threads:
+// a large block's reduction is bound by how many of its matrix rows are in
+// flight at once.
 #include "ftlm_device.h"
 #include "ftlm_launch.h"
 
@@ -14,8 +17,9 @@ namespace ftlm {
 
 namespace {
 
-constexpr int kT = 256;
+constexpr int kBigN = 512;
 
+template <int kT>
 __device__ double block_sum(double v, double* red) {
   const int tid = threadIdx.x;
   red[tid] = v;
@@ -95,18 +99,25 @@ __device__ bool tridiag_ql(double* d, double* e, int n) {
   return true;
 }
 
+// kT threads per block; the launch for kT handles the blocks of its size
+// class (big: n >= kBigN) and skips the others.
+template <int kT>
 __global__ void __launch_bounds__(kT) k_herm_eig(const int64_t* __restrict__ moff, const int64_t* __restrict__ dims,
                                                  double* mats, double* work, double* __restrict__ eig_out,
                                                  const int64_t* __restrict__ eoff) {
+  // the Householder vector v and p = A22 v in LDS when they fit (always for
+  // the small class), else in the block's work area
+  constexpr int kLdsN = kT > 256 ? 2048 : kBigN;
   __shared__ double red[kT];
+  __shared__ double shv[2 * kLdsN], shp[2 * kLdsN];
   const int b = blockIdx.x;
   const int64_t n = dims[b];
-  if (n == 0) return;
+  if (n == 0 || (n >= kBigN) != (kT > 256)) return;
   double* A = mats + 2 * moff[b];
   // work: per block v (n complex), p (n complex), d (n), e (n) = 6n doubles
   double* wk = work + 6 * eoff[b];
-  double* V = wk;           // 2n
-  double* P = wk + 2 * n;   // 2n
+  double* V = n <= kLdsN ? shv : wk;          // 2n
+  double* P = n <= kLdsN ? shp : wk + 2 * n;  // 2n
   double* D = wk + 4 * n;   // n
   double* E = wk + 5 * n;   // n
   const int tid = threadIdx.x;
@@ -117,7 +128,7 @@ __global__ void __launch_bounds__(kT) k_herm_eig(const int64_t* __restrict__ mof
       const double* z = a(i, k);
       part += z[0] * z[0] + z[1] * z[1];
     }
-    const double xn = sqrt(block_sum(part, red));
+    const double xn = sqrt(block_sum<kT>(part, red));
     if (xn == 0.0) continue;
     const double x0r = a(k + 1, k)[0], x0i = a(k + 1, k)[1];
     const double ax0 = hypot_safe(x0r, x0i);
@@ -134,7 +145,7 @@ __global__ void __launch_bounds__(kT) k_herm_eig(const int64_t* __restrict__ mof
       V[2 * i + 1] = vi;
       part += vr * vr + vi * vi;
     }
-    const double vn = sqrt(block_sum(part, red));
+    const double vn = sqrt(block_sum<kT>(part, red));
     if (vn == 0.0) continue;
     for (int64_t i = k + 1 + tid; i < n; i += kT) {
       V[2 * i] /= vn;
@@ -155,7 +166,7 @@ __global__ void __launch_bounds__(kT) k_herm_eig(const int64_t* __restrict__ mof
       P[2 * i + 1] = si;
       kr += V[2 * i] * sr + V[2 * i + 1] * si;  // Re(conj(v_i) p_i)
     }
-    const double K = block_sum(kr, red);
+    const double K = block_sum<kT>(kr, red);
     for (int64_t i = k + 1 + tid; i < n; i += kT) {
       P[2 * i] -= K * V[2 * i];
       P[2 * i + 1] -= K * V[2 * i + 1];
@@ -207,7 +218,8 @@ __global__ void __launch_bounds__(kT) k_herm_eig(const int64_t* __restrict__ mof
 void launch_herm_eig(const int64_t* moff, const int64_t* dims, int nblk, double* mats, double* work,
                      double* eig_out, const int64_t* eoff, hipStream_t s) {
   if (nblk <= 0) return;
-  k_herm_eig<<<nblk, kT, 0, s>>>(moff, dims, mats, work, eig_out, eoff);
+  k_herm_eig<256><<<nblk, 256, 0, s>>>(moff, dims, mats, work, eig_out, eoff);
+  k_herm_eig<1024><<<nblk, 1024, 0, s>>>(moff, dims, mats, work, eig_out, eoff);
 }
 
 }  // namespace ftlm
