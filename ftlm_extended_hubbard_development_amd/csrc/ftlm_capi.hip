@@ -1142,11 +1142,21 @@ int pack_width(const PackState& p, int width) {
   return W;
 }
 
-// items (entry, column group) in decreasing cost; ties keep entry order
-std::vector<PackItem> pack_items(const PackState& p, int W) {
+// items (entry, column group) in decreasing cost; ties keep entry order.
+// stride > 1 keeps every stride-th entry in decreasing cost (the first, most
+// costly one included): a sample with the pack's mix of block sizes.
+std::vector<PackItem> pack_items(const PackState& p, int W, int stride = 1) {
+  std::vector<char> keep(p.entries.size(), 1);
+  if (stride > 1) {
+    std::vector<int32_t> order(p.entries.size());
+    for (size_t e = 0; e < order.size(); ++e) order[e] = (int32_t)e;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return p.cost[a] > p.cost[b]; });
+    for (size_t i = 0; i < order.size(); ++i) keep[order[i]] = i % stride == 0;
+  }
   std::vector<PackItem> items;
   for (size_t e = 0; e < p.entries.size(); ++e)
-    for (int c0 = 0; c0 < p.R; c0 += W) items.push_back({(int32_t)e, c0, std::min(W, p.R - c0), 0});
+    if (keep[e])
+      for (int c0 = 0; c0 < p.R; c0 += W) items.push_back({(int32_t)e, c0, std::min(W, p.R - c0), 0});
   std::stable_sort(items.begin(), items.end(), [&](const PackItem& a, const PackItem& b) {
     return p.cost[a.entry] * a.w > p.cost[b.entry] * b.w;
   });
@@ -1165,9 +1175,9 @@ size_t pack_slot_limit(ftlm_ctx* c) {
   return std::min(std::max(kPackSlotLimit, avail / 2), (size_t)128 << 30);
 }
 
-PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax) {
+PackArgs pack_args(ftlm_ctx* c, int W, int* grid, int kmax, int sample = 1) {
   PackState& p = c->pack;
-  const std::vector<PackItem> items = pack_items(p, W);
+  const std::vector<PackItem> items = pack_items(p, W, sample);
   const int ncols = (int)p.entries.size() * p.R;
   const int rcap = std::max(p.rcap, (ncols + 1023) / 1024 * 1024);
   p.rcap = rcap;
@@ -1308,10 +1318,11 @@ double pack_run(ftlm_ctx* c, int width, int wg_per_cu, int method) {
   return ms;
 }
 
-// Timing of the persistent kernel alone over `kmax` steps (tuning).
-double pack_time(ftlm_ctx* c, int W, int wg_per_cu, int kmax) {
+// Timing of the persistent kernel alone over `kmax` steps on every
+// sample-th entry (tuning).
+double pack_time(ftlm_ctx* c, int W, int wg_per_cu, int kmax, int sample) {
   int grid = c->cus * wg_per_cu;
-  const PackArgs a = pack_args(c, W, &grid, kmax);
+  const PackArgs a = pack_args(c, W, &grid, kmax, sample);
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
@@ -2008,12 +2019,20 @@ int ftlm_pack_tune(ftlm_ctx* ctx, int tune_steps, int* best_width, int* best_wg_
     PackState& p = ctx->pack;
     if (!p.open || p.entries.empty()) throw Error(FTLM_ERR_STATE, "empty pack");
     const int cap = ncand ? *ncand : 0;
+    // A large pack is timed on a sample of its entries (every stride-th in
+    // decreasing cost), as long as the widest column groups still give 4
+    // items per workgroup of the largest grid: each candidate's timing then
+    // stays a small share of the run (eight candidates of two steps over the
+    // whole pack cost a fifth of an 80-step run).
+    const int64_t groups = (int64_t)p.entries.size() * ((p.R + 63) / 64);
+    const int64_t need = (int64_t)4 * ctx->cus * 2;
+    const int stride = (int)std::max<int64_t>(1, std::min<int64_t>(groups / need, 1 << 20));
     int n = 0, bw = 0, bg = 0;
     double bms = 0.0;
     for (int W : {8, 16, 32, 64}) {
       if (W > 16 && W >= 2 * p.R) continue;
       for (int wg : {1, 2}) {  // 4 per CU measured no better (throughput-bound)
-        const double ms = pack_time(ctx, W, wg, tune_steps > 0 ? tune_steps : 4);
+        const double ms = pack_time(ctx, W, wg, tune_steps > 0 ? tune_steps : 4, stride);
         if (n < cap) {
           if (cand_width) cand_width[n] = W;
           if (cand_wg) cand_wg[n] = wg;

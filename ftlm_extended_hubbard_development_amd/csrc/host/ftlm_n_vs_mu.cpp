@@ -381,7 +381,9 @@ int main(int argc, char** argv) {
         // vectors): enough to rank (width, grid), a small share of the run
         check(ftlm_pack_tune(ctx.c, 2, &bw, &bg, &bms, &n, nullptr, nullptr, nullptr), "pack tuning");
         tuned = true;
-        tj_rec("\"method\":\"pack_tune\",\"width\":" + std::to_string(bw) + ",\"wg_per_cu\":" + std::to_string(bg));
+        std::ostringstream rec;
+        rec << "\"method\":\"pack_tune\",\"width\":" << bw << ",\"wg_per_cu\":" << bg << ",\"ms\":" << ms_since(t0);
+        tj_rec(rec.str());
       }
       check(ftlm_pack_run(ctx.c, o.pack_width, o.pack_wg, pack_method), "packed FTLM");
       for (std::size_t pos : pending) {

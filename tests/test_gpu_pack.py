@@ -186,6 +186,26 @@ def test_pack_tune(pkg, orc, eng3):
         assert np.array_equal(e, oe) and np.array_equal(w, ow)
 
 
+def test_pack_tune_sampled(pkg, orc, eng3):
+    """A pack large enough (4,200 entries of one 64-column group each) for the
+    tuner to time every second entry in decreasing cost on 256 CUs: the pick
+    is one of the candidates and the run that follows is bit-exact."""
+    copies = 600
+    ents = [(TWISTS[0], c) for c in BLOCKS_3x3 for _ in range(copies)]
+    idx = fill_pack(pkg, eng3, ents, 8, 40)
+    assert len(idx) == len(ents)
+    (bw, bg), ms, cand = eng3.pack_tune(2)
+    say(f"sampled pack tune: {cand} -> W{bw}/wg{bg}")
+    assert f"W{bw}/wg{bg}" in cand and ms > 0
+    eng3.pack_run(0, 0, method=0)
+    md = dict(MODEL_3x3, phix=TWISTS[0][0], phiy=TWISTS[0][1])
+    for k, case in enumerate(BLOCKS_3x3):
+        oe, ow = oracle_result(orc, md, case, 8, 40)
+        for i in idx[k * copies:(k + 1) * copies]:
+            e, w = eng3.pack_result(i)
+            assert np.array_equal(e, oe) and np.array_equal(w, ow), (case, i)
+
+
 def test_pack_errors(pkg, eng3):
     eng3.pack_begin(8, 40)
     with pytest.raises(pkg.FtlmError):
