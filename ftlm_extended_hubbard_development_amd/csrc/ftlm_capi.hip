@@ -960,6 +960,16 @@ void tune_placement(ftlm_ctx* c, int R) {
     cand.push_back(std::make_unique<DevBuf>());
     cand.back()->swap(c->vec[i]);
   }
+  // an error before the choice gives the context its own buffers back
+  struct Restore {
+    ftlm_ctx* c;
+    std::vector<std::unique_ptr<DevBuf>>& cand;
+    bool armed = true;
+    ~Restore() {
+      if (armed)
+        for (int i = 0; i < 3; ++i) c->vec[i].swap(*cand[i]);
+    }
+  } restore{c, cand};
   for (int i = 3; i < c->place_candidates; ++i) {
     if (fr < reserve + cap) break;
     void* p = nullptr;
@@ -1022,6 +1032,7 @@ void tune_placement(ftlm_ctx* c, int R) {
     HIPCHECK(hipMemcpyAsync(cand[order[0]]->p, cand[0]->p, need, hipMemcpyDeviceToDevice, s));
   sync(c);
   const int pick[3] = {order[0], order[1], out};
+  restore.armed = false;
   for (int i = 0; i < 3; ++i) c->vec[i].swap(*cand[pick[i]]);
   c->place_ms = ms;
   c->place_pick.assign(pick, pick + 3);
