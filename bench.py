@@ -255,9 +255,11 @@ def memory_ceiling(eng, R, rows=10353252, k=35, reps=3):
     (ftlm_memprobe): stream read and copy of rows x R complex doubles, and a
     gather writing each row as the sum of k hashed rows (k = 35 ~ the 34.13
     hop targets + the row itself of the (8,8) block, one wavefront per row
-    like k_apply_hl)."""
+    like k_apply_hl), the same gather inside each row's 12,870-row window
+    (one canonical-up group: where the down-spin hops land), and half of it
+    inside the window (the apply's mix of down- and up-spin hops)."""
     out = {"rows": rows, "width": R, "row_bytes": R * 16}
-    for kind, name in ((0, "read"), (1, "copy"), (2, "gather")):
+    for kind, name in ((0, "read"), (1, "copy"), (2, "gather"), (3, "gather_window"), (4, "gather_half_window")):
         ms, by = eng.memprobe(kind, rows, R, k, reps)
         out[name] = {"ms": round(ms, 3), "GB": round(by / 1e9, 2), "TBps": round(by / ms / 1e9, 3)}
         print(f"# ceiling {name}: {ms:.2f} ms, {by / ms / 1e9:.2f} TB/s", file=sys.stderr, flush=True)

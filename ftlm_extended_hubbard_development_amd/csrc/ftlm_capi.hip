@@ -2146,8 +2146,9 @@ int ftlm_memprobe(ftlm_ctx* ctx, int kind, int64_t rows, int width, int k, int r
                   double* bytes_per_launch) {
   return guarded([&] {
     if (!ctx || !ms_per_launch || !bytes_per_launch) throw Error(FTLM_ERR_ARG, "null argument");
-    if (kind < 0 || kind > 2) throw Error(FTLM_ERR_ARG, "kind must be 0 (read), 1 (copy) or 2 (gather)");
-    if (rows < 1 || width < 1 || width > 256 || reps < 1 || (kind == 2 && k < 1))
+    if (kind < 0 || kind > 4)
+      throw Error(FTLM_ERR_ARG, "kind must be 0 (read), 1 (copy), 2 (gather), 3 (windowed gather) or 4 (mixed gather)");
+    if (rows < 1 || width < 1 || width > 256 || reps < 1 || (kind >= 2 && k < 1))
       throw Error(FTLM_ERR_ARG, "rows, reps and k must be positive, width in [1, 256]");
     HIPCHECK(hipSetDevice(ctx->device));
     const size_t bytes = (size_t)rows * width * 16;
@@ -2169,7 +2170,7 @@ int ftlm_memprobe(ftlm_ctx* ctx, int kind, int64_t rows, int width, int k, int r
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms_per_launch = ms / reps;
-    *bytes_per_launch = kind == 0 ? (double)bytes : kind == 1 ? 2.0 * bytes : (double)bytes * (k + 1);
+    *bytes_per_launch = kind == 0 ? (double)bytes : kind == 1 ? 2.0 * bytes : (double)bytes * (k + 1);  // gathers: k rows in, 1 out
   });
 }
 

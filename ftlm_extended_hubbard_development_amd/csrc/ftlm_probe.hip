@@ -8,7 +8,13 @@
 //   kind 2  random-row gather: each output row adds `k` source rows chosen by
 //           a fixed hash of (row, j) and writes the sum -- one wavefront per
 //           row with the apply's lane layout, k rows in flight in groups of
-//           four, the XCD-contiguous row ranges of the apply.
+//           four, the XCD-contiguous row ranges of the apply;
+//   kind 3  the same gather with every source row drawn inside the output
+//           row's window of kProbeWindow rows (12,870 = C(16,8): one
+//           canonical-up group of the half-filled 4x4 block, 26 MB at 128
+//           columns, where the apply's down-spin hops land);
+//   kind 4  the same gather with sources alternately in the window and over
+//           the whole vector (the apply's mix of down- and up-spin hops).
 // The sums are written so that no load is dead code.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,6 +25,7 @@ namespace ftlm {
 namespace {
 
 constexpr int kProbeXcd = 8;
+constexpr int64_t kProbeWindow = 12870;
 
 __device__ __forceinline__ uint64_t probe_hash(uint64_t z) {
   z ^= z >> 33;
@@ -83,18 +90,25 @@ __global__ void __launch_bounds__(256) k_probe_stream(int64_t rows, int width, i
 
 template <int VPL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(VPL <= 2 ? 8 : 4)))
-k_probe_gather(int64_t rows, int width, int k, const double2* __restrict__ x, double2* __restrict__ y) {
+k_probe_gather(int64_t rows, int width, int k, int mode, const double2* __restrict__ x, double2* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   int64_t r, end, step;
   probe_rows(rows, &r, &end, &step);
   for (; r < end; r += step) {
+    // the row's window [w0, w0 + wl), inside [0, rows) since w0 <= r < rows
+    const int64_t w0 = r / kProbeWindow * kProbeWindow;
+    const int64_t wl = rows - w0 < kProbeWindow ? rows - w0 : kProbeWindow;
     double2 acc[VPL];
 #pragma unroll
     for (int v = 0; v < VPL; ++v) acc[v] = make_double2(0.0, 0.0);
     for (int j = 0; j < k; j += 4) {
       int64_t src[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) src[q] = (int64_t)(probe_hash((uint64_t)r * 131 + (uint64_t)(j + q)) % (uint64_t)rows);
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t h = probe_hash((uint64_t)r * 131 + (uint64_t)(j + q));
+        const bool local = mode == 1 || (mode == 2 && ((j + q) & 1) == 0);
+        src[q] = local ? w0 + (int64_t)(h % (uint64_t)wl) : (int64_t)(h % (uint64_t)rows);
+      }
       double2 t[4][VPL];
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -121,8 +135,8 @@ k_probe_gather(int64_t rows, int width, int k, const double2* __restrict__ x, do
 
 template <int VPL>
 void probe_launch(int kind, int64_t rows, int width, int k, int grid, const double2* x, double2* y, hipStream_t s) {
-  if (kind == 2)
-    k_probe_gather<VPL><<<grid, 256, 0, s>>>(rows, width, k, x, y);
+  if (kind >= 2)
+    k_probe_gather<VPL><<<grid, 256, 0, s>>>(rows, width, k, kind - 2, x, y);
   else
     k_probe_stream<VPL><<<grid, 256, 0, s>>>(rows, width, kind == 1 ? 1 : 0, x, y);
 }

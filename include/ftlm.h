@@ -286,10 +286,12 @@ int ftlm_exact_blocks(ftlm_ctx* ctx, int nblk, const int32_t* mx, const int32_t*
  * hop metadata and arithmetic, on `rows` rows of `width` (<= 256) complex
  * doubles. kind 0 streams the rows in (read only), 1 copies them, 2 writes
  * every row as the sum of `k` source rows chosen by a fixed hash (one
- * wavefront per row, like the batched apply). Allocates two buffers of
- * rows * width * 16 bytes for the call, times `reps` launches after one
- * warm-up launch; *ms_per_launch and *bytes_per_launch (read + written
- * bytes: rows * width * 16 times 1, 2 or k + 1) are returned. */
+ * wavefront per row, like the batched apply); 3 draws the k rows inside the
+ * row's window of 12,870 rows (one canonical-up group of the half-filled
+ * 4x4 block), 4 alternately inside the window and over all rows. Allocates
+ * two buffers of rows * width * 16 bytes for the call, times `reps` launches
+ * after one warm-up launch; *ms_per_launch and *bytes_per_launch (read +
+ * written bytes: rows * width * 16 times 1, 2 or k + 1) are returned. */
 int ftlm_memprobe(ftlm_ctx* ctx, int kind, int64_t rows, int width, int k, int reps, double* ms_per_launch,
                   double* bytes_per_launch);
 
