@@ -205,7 +205,9 @@ int ftlm_set_twist(ftlm_ctx* ctx, double phix, double phiy);
  *                     ftlm_ftlm_block returns them.
  *   ftlm_pack_tune    times the (width, wg_per_cu) candidates on the current
  *                     pack over `tune_steps` steps and keeps the fastest for
- *                     later runs with 0 arguments; a large pack is timed on
+ *                     later runs with 0 arguments (widths 8, 16, 32, 64 with
+ *                     1 and 2 workgroups per CU, widening only while the
+ *                     wider width wins); a large pack is timed on
  *                     every k-th entry in decreasing cost, k chosen so that
  *                     the sample still holds 8 * CUs column groups of 64
  *                     (best_ms and cand_ms are those of the sample).
